@@ -1,0 +1,248 @@
+"""Pins the CPU oracle before it is trusted as the checker of the HIP kernels.
+
+Each test checks an oracle routine against something that does not share its
+formula: the reference's own known-answer vector, textbook brute force, closed forms
+or invariants chosen so that a dropped term, a wrong sign or a transposed operand
+fails at least one of them.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# The nine occupation patterns of the reference's
+# tests/test_bs_permanent_calculators.py:88-122 (m = 4).
+REFERENCE_PATTERNS = [
+    ([1, 1, 1, 1], [1, 1, 1, 1]),
+    ([1, 1, 1, 0], [1, 0, 1, 1]),
+    ([2, 1, 0, 0], [0, 1, 1, 1]),
+    ([1, 1, 1, 0], [2, 1, 0, 0]),
+    ([2, 1, 1, 0], [1, 1, 0, 2]),
+    ([0, 0, 4, 0], [1, 1, 0, 2]),
+    ([1, 1, 0, 2], [0, 1, 3, 0]),
+    ([0, 2, 0, 2], [0, 0, 4, 0]),
+    ([0, 0, 0, 0], [0, 0, 0, 0]),
+]
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+def test_haar_is_unitary_and_seeded(oracle):
+    U = oracle.haar_unitary(12, 7)
+    assert np.allclose(U.conj().T @ U, np.eye(12), atol=1e-13)
+    assert np.array_equal(U, oracle.haar_unitary(12, 7))
+    assert not np.array_equal(U, oracle.haar_unitary(12, 8))
+
+
+def test_effective_matrix_definition(oracle):
+    U = oracle.haar_unitary(5, 1)
+    s, t = [2, 0, 1, 0, 1], [0, 3, 0, 1, 0]
+    M = oracle.effective_matrix(U, s, t)
+    rows = np.repeat(np.arange(5), t)
+    cols = np.repeat(np.arange(5), s)
+    assert np.array_equal(M, U[np.ix_(rows, cols)])
+    assert oracle.effective_matrix(U, [0] * 5, t).shape == (0, 0)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7])
+def test_classic_equals_permutation_sum(oracle, n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    assert rel(oracle.perm_classic_matrix(A), oracle.perm_by_permutations(A)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 4, 8])
+def test_classic_closed_forms(oracle, n):
+    J = np.ones((n, n), dtype=complex)
+    assert rel(oracle.perm_classic_matrix(J), math.factorial(n)) < 1e-13
+    if n > 1:
+        assert rel(oracle.perm_classic_matrix(J - np.eye(n)), oracle.derangements(n)) < 1e-13
+    d = np.arange(1, n + 1) * (1 + 0.5j)
+    assert rel(oracle.perm_classic_matrix(np.diag(d)), np.prod(d)) < 1e-13
+    P = np.eye(n)[np.random.default_rng(0).permutation(n)]
+    assert rel(oracle.perm_classic_matrix(P), 1.0) < 1e-15
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("n", [1, 2, 3, 6, 8])
+def test_glynn_equals_classic(oracle, n, mode):
+    A = oracle.haar_unitary(2 * n, 100 + n)[:n, :n]
+    assert rel(oracle.perm_glynn_matrix(A, mode), oracle.perm_classic_matrix(A)) < 1e-11
+
+
+@pytest.mark.parametrize("n", [10, 16, 21])
+def test_glynn_closed_forms_large(oracle, n):
+    rng = np.random.default_rng(n)
+    u = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    # Rank one: every Glynn term is prod_j (sum_i delta_i u_i) v_j; heavy cancellation.
+    expect = math.factorial(n) * np.prod(u) * np.prod(v)
+    assert rel(oracle.perm_glynn_matrix(np.outer(u, v), 1), expect) < 1e-10
+    J = np.ones((n, n), dtype=complex)
+    assert rel(oracle.perm_glynn_matrix(J, 1), math.factorial(n)) < 1e-12
+    assert rel(oracle.perm_glynn_matrix(J - np.eye(n), 1), oracle.derangements(n)) < 1e-12
+
+
+def test_glynn_invariances(oracle):
+    n = 9
+    rng = np.random.default_rng(3)
+    A = oracle.haar_unitary(18, 3)[:n, :n]
+    p = oracle.perm_glynn_matrix(A, 1)
+    assert rel(oracle.perm_glynn_matrix(A.T, 1), p) < 1e-12
+    rp, cp = rng.permutation(n), rng.permutation(n)
+    assert rel(oracle.perm_glynn_matrix(A[rp][:, cp], 1), p) < 1e-12
+    c = 0.7 - 0.4j
+    assert rel(oracle.perm_glynn_matrix(c * A, 1), c ** n * p) < 1e-12
+    B = oracle.haar_unitary(8, 4)[:4, :4]
+    blk = np.zeros((n + 4, n + 4), dtype=complex)
+    blk[:n, :n], blk[n:, n:] = A, B
+    assert rel(oracle.perm_glynn_matrix(blk, 1), p * oracle.perm_classic_matrix(B)) < 1e-11
+
+
+@pytest.mark.parametrize("s,t", REFERENCE_PATTERNS)
+def test_guan_calculators_reference_patterns(oracle, s, t):
+    U = oracle.haar_unitary(4, 11)
+    expect = oracle.perm_classic(U, s, t)
+    assert abs(oracle.perm_ryser(U, s, t) - expect) < 1e-12 * max(1, abs(expect))
+    assert abs(oracle.perm_chin_huh(U, s, t) - expect) < 1e-12 * max(1, abs(expect))
+    if sum(s):
+        assert abs(oracle.perm_glynn(U, s, t, 0) - expect) < 1e-12 * max(1, abs(expect))
+    else:
+        assert oracle.perm_glynn(U, s, t, 0) == 1  # empty effective matrix -> 1
+
+
+def test_guan_calculators_random_multiplicities(oracle):
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        m = int(rng.integers(3, 7))
+        n = int(rng.integers(1, 7))
+        U = oracle.haar_unitary(m, 200 + trial)
+        s = np.bincount(rng.integers(0, m, n), minlength=m)
+        t = np.bincount(rng.integers(0, m, n), minlength=m)
+        expect = oracle.perm_classic(U, s, t)
+        assert abs(oracle.perm_ryser(U, s, t) - expect) < 1e-12
+        assert abs(oracle.perm_chin_huh(U, s, t) - expect) < 1e-12
+
+
+# Cases of the reference's tests/test_bs_submatrices_permanent_calculators.py:41-73.
+SUBMATRIX_CASES = [
+    ([1, 1, 1, 1], [0, 1, 1, 1]),
+    ([1, 3, 0, 0], [0, 1, 1, 1]),
+    ([1, 0, 0, 0], [0, 0, 0, 0]),
+]
+
+
+@pytest.mark.parametrize("formula", [0, 1])
+@pytest.mark.parametrize("s,t", SUBMATRIX_CASES)
+def test_cc_submatrices_against_classic_minors(oracle, s, t, formula):
+    U = oracle.haar_unitary(4, 21)
+    got = oracle.cc_submatrices(U, s, t, formula)
+    for i in range(4):
+        if s[i] == 0:
+            assert got[i] == 0
+            continue
+        sm = list(s)
+        sm[i] -= 1
+        expect = oracle.perm_classic(U, sm, t)
+        assert abs(got[i] - expect) < 1e-12, (i, got[i], expect)
+
+
+def test_cc_submatrices_random_and_laplace(oracle):
+    rng = np.random.default_rng(9)
+    for trial in range(15):
+        m = int(rng.integers(3, 8))
+        k = int(rng.integers(2, 7))
+        U = oracle.haar_unitary(m, 300 + trial)
+        s = np.bincount(rng.integers(0, m, k), minlength=m)
+        t = np.bincount(rng.integers(0, m, k - 1), minlength=m)
+        minors = oracle.cc_submatrices(U, s, t, 0)
+        for i in np.nonzero(s)[0]:
+            sm = s.copy()
+            sm[i] -= 1
+            assert abs(minors[i] - oracle.perm_classic(U, sm, t)) < 1e-12
+        # Laplace expansion along the added output row e_o.
+        for o in range(m):
+            to = t.copy()
+            to[o] += 1
+            lap = sum(s[i] * minors[i] * U[o, i] for i in range(m))
+            assert abs(lap - oracle.perm_classic(U, s, to)) < 1e-12
+
+
+def test_minors_laplace_and_classic(oracle):
+    for n, seed in [(2, 1), (5, 2), (7, 3), (12, 4)]:
+        A = oracle.haar_unitary(2 * n, seed)[:n, :n]
+        G = oracle.perm_minors(A)
+        p = oracle.perm_glynn_matrix(A, 1)
+        for i in range(n):
+            assert rel(np.sum(A[i] * G[i]), p) < 1e-10
+            assert rel(np.sum(A[:, i] * G[:, i]), p) < 1e-10
+        if n <= 7:
+            for i in range(n):
+                for j in range(n):
+                    sub = np.delete(np.delete(A, i, 0), j, 1)
+                    assert abs(G[i, j] - oracle.perm_classic_matrix(sub)) < 1e-13
+
+
+def test_gccb_pmf_matches_direct_permanents(oracle):
+    m = 6
+    U = oracle.haar_unitary(m, 42)
+    cur = [1, 1, 0, 1, 0, 0]
+    r = [0, 2, 0, 0, 0, 0]
+    pmf, _ = oracle.gccb_pmf(U, cur, r)
+    direct = []
+    for o in range(m):
+        t = list(r)
+        t[o] += 1
+        direct.append(abs(oracle.perm_classic(U, cur, t)) ** 2)
+    direct = np.asarray(direct) / sum(direct)
+    assert np.allclose(pmf, direct, atol=1e-14, rtol=1e-12)
+    assert abs(pmf.sum() - 1) < 1e-14
+
+
+def test_exact_distribution_known_answer(oracle):
+    """The reference's only deterministic known-answer test."""
+    with open(os.path.join(HERE, "golden", "exact_distribution_kat.json")) as f:
+        kat = json.load(f)
+    U = np.asarray(kat["interferometer_real"], dtype=np.complex128)
+    dist = oracle.uniform_losses_distribution(U, kat["initial_state"], kat["uniform_transmissivity"])
+    assert len(dist) == len(kat["expected_uniform_losses_distribution"])
+    assert np.allclose(dist, kat["expected_uniform_losses_distribution"])
+    assert abs(sum(dist) - 1) < kat["probability_sum_tolerance"]
+    # Same with Ryser and the classic calculator.
+    for perm in (oracle.perm_ryser, oracle.perm_classic):
+        d2 = oracle.uniform_losses_distribution(U, kat["initial_state"], 0.8, perm)
+        assert np.allclose(d2, kat["expected_uniform_losses_distribution"])
+
+
+def test_exact_distribution_sums_to_one_haar(oracle):
+    U = oracle.haar_unitary(4, 77)
+    for init in ([1, 1, 1, 0], [2, 1, 0, 0]):
+        d = oracle.uniform_losses_distribution(U, init, 0.6, oracle.perm_ryser)
+        assert abs(sum(d) - 1) < 1e-12
+
+
+@pytest.mark.parametrize("s", [[1, 1, 1], [2, 0, 3], [0, 0, 0], [1, 2, 1, 3]])
+def test_guan_sequence_is_mixed_radix_gray(oracle, s):
+    seq = oracle.guan_sequence(s)
+    total = int(np.prod(np.asarray(s) + 1))
+    assert len(seq) == total
+    assert len({tuple(r) for r in seq}) == total
+    assert np.all(seq >= 0) and np.all(seq <= np.asarray(s))
+    d = np.abs(np.diff(seq, axis=0))
+    assert np.all(d.sum(axis=1) == 1)
+
+
+def test_gcca_layer_pmf_normalises_to_fixed_loss_marginal(oracle):
+    """Summed over the first layer, GCC-A weights reproduce the one-particle marginal
+    sum_i s_i |U[o][i]|^2 / n (closed form for k = 1)."""
+    U = oracle.haar_unitary(5, 5)
+    s = [1, 2, 0, 1, 0]
+    pmf = oracle.gcca_layer_pmf(U, s, [0] * 5)
+    marg = np.array([sum(s[i] * abs(U[o, i]) ** 2 for i in range(5)) for o in range(5)]) / sum(s)
+    assert np.allclose(pmf / pmf.sum(), marg, atol=1e-13)
