@@ -1,0 +1,185 @@
+/*
+ * theboss_amd: C ABI of the MI355X (gfx950) permanent kernels.
+ *
+ * Conventions shared by every entry point:
+ *   - Array arguments are DEVICE pointers (e.g. torch tensor data_ptr()); complex
+ *     values are interleaved double pairs (torch.complex128 / C99 double _Complex).
+ *   - Matrices are row-major. The interferometer U is indexed U[output_mode][input_mode]
+ *     (theboss convention, ryser_permanent_calculator.py:61-64).
+ *   - `stream` is a hipStream_t (0 = the null stream). Calls are asynchronous on it.
+ *   - Scratch memory is passed in by the caller (`workspace`, size from the matching
+ *     *_workspace_size query); the library never allocates or frees device memory
+ *     inside an entry point, so calls can be captured in a hipGraph.
+ *   - Return value: TB_OK or a negative TB_E* code; tb_last_error() then holds a
+ *     message for the calling thread. Arguments are validated on the host before any
+ *     launch. No C++ exception crosses this boundary.
+ * Reference paths below are relative to the theboss repository root.
+ */
+#ifndef THEBOSS_AMD_H
+#define THEBOSS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TB_OK 0
+#define TB_EINVAL (-1)      /* bad shape / argument */
+#define TB_ERANGE (-2)      /* size beyond the compiled maximum */
+#define TB_EHIP (-3)        /* HIP runtime error */
+#define TB_EWORKSPACE (-4)  /* workspace too small */
+
+#define TB_PREC_FP64 0          /* FP64 column sums, products and accumulation */
+#define TB_PREC_FP32_COLSUM 1   /* FP32 column sums and products, FP64 accumulation */
+
+#define TB_FORMULA_RYSER 0
+#define TB_FORMULA_CHIN_HUH 1
+
+#ifndef TB_GLYNN_MAX_N
+#define TB_GLYNN_MAX_N 48   /* largest square matrix for tb_glynn_batch */
+#endif
+#define TB_SWEEP_MAX_K 40   /* largest particle count for the binary C&C sweep */
+#define TB_GUAN_MAX_ROWS 48 /* distinct output modes for the mixed-radix kernels */
+#define TB_GUAN_MAX_COLS 32 /* distinct input modes for the mixed-radix kernels */
+
+typedef struct {
+  double re, im;
+} tb_complex128;
+
+/* Library version (major*10000 + minor*100 + patch). No reference counterpart. */
+int32_t tb_version(void);
+
+/* Message of the last failed call on this thread ("" if none). No reference
+ * counterpart: the reference raises Python exceptions
+ * (bs_permanent_calculator_base.py:179-180). */
+const char *tb_last_error(void);
+
+/* Number of HIP devices visible, or a negative TB_E* code. */
+int32_t tb_device_count(void);
+
+/* ---------------------------------------------------------------------------------
+ * Glynn permanents of a batch of n x n matrices.
+ * Replaces GlynnGrayPermanentCalculator.compute_permanent
+ * (theboss/boson_sampling_utilities/permanent_calculators/glynn_gray_permanent_calculator.py:41-71)
+ * applied to already-gathered effective matrices (boson_sampling_utilities.py:595-626).
+ *   A          batch matrices, matrix b at A + b*matrix_stride (elements), row-major n x n
+ *   precision  TB_PREC_FP64 or TB_PREC_FP32_COLSUM
+ *   out        batch permanents
+ * n == 0 yields 1 for every matrix (glynn_gray_permanent_calculator.py:52-53).
+ * ------------------------------------------------------------------------------- */
+size_t tb_glynn_workspace_size(int32_t n, int32_t batch, int32_t precision);
+int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t batch, int64_t matrix_stride,
+                       int32_t precision, tb_complex128 *out, void *workspace,
+                       size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * Multiplicity-aware permanents over Guan codes (reflected mixed-radix Gray codes).
+ * Replaces RyserPermanentCalculator.compute_permanent and
+ * ChinHuhPermanentCalculator.compute_permanent
+ * (bs_permanent_calculator_base.py:166-209, ryser_permanent_calculator.py:45-64,
+ *  chin_huh_permanent_calculator.py:38-59) for a batch of (input, output) states on
+ * one interferometer.
+ *   Item b uses output modes out_modes[b*out_max + j] with multiplicities
+ *   out_mult[b*out_max + j] (t_j; 0 pads) and input modes in_modes[b*in_max + d] with
+ *   multiplicities in_mult[b*in_max + d] (s_d; 0 pads). The effective matrix has
+ *   rows U[out_mode] repeated t times and columns in_mode repeated s times.
+ *   formula    TB_FORMULA_RYSER or TB_FORMULA_CHIN_HUH
+ *   out        batch permanents (same value as the reference, which enumerates the
+ *              input side: sum over r <= s).
+ * ------------------------------------------------------------------------------- */
+int32_t tb_guan_perm_batch(const tb_complex128 *U, int32_t ldu, const int32_t *out_modes,
+                           const int32_t *out_mult, int32_t out_max, const int32_t *in_modes,
+                           const int32_t *in_mult, int32_t in_max, int32_t batch,
+                           int32_t formula, tb_complex128 *out, void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * Clifford & Clifford submatrices permanents: all input-particle-deleted minors in one
+ * sweep. Replaces BSCCRyserSubmatricesPermanentCalculator.compute_permanents
+ * (bs_submatrices_permanent_calculator_base.py:150-189,
+ *  bs_cc_ryser_submatrices_permanent_calculator.py:73-119).
+ * Same item description as tb_guan_perm_batch. minors[b*in_max + d] receives
+ * perm(U[t, s - e_{in_modes[b,d]}]) (0 where in_mult is 0). Every item of a call must
+ * have the same sum(s) = k and sum(t) = k - 1 (the caller passes k, and
+ * collision_free = 1 when every s_d <= 1). k == 1 returns s itself
+ * (bs_submatrices_permanent_calculator_base.py:156-158). Collision-free inputs run the
+ * binary Gray sweep; others run the mixed-radix kernel.
+ * ------------------------------------------------------------------------------- */
+size_t tb_cc_workspace_size(int32_t k, int32_t out_max, int32_t in_max, int32_t batch);
+int32_t tb_cc_submatrices_batch(const tb_complex128 *U, int32_t ldu, const int32_t *out_modes,
+                                const int32_t *out_mult, int32_t out_max,
+                                const int32_t *in_modes, const int32_t *in_mult, int32_t in_max,
+                                int32_t batch, int32_t k, int32_t collision_free,
+                                tb_complex128 *minors, void *workspace, size_t workspace_bytes,
+                                void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * Binary C&C sweep on pre-gathered column data (the building block shared by the
+ * submatrices calculator, the gradient and the batched sampler).
+ *   Vt      item b, particle nu, row j at Vt[b*item_stride + nu*ldv + j]:
+ *           Vt[nu][j] = U[out_j][in_nu] for the k input particles (Gray bits) and the
+ *           k-1 output particles (rows, repeated for multiplicities).
+ *   minors  minors[b*k + nu] = perm of the (k-1)x(k-1) matrix without particle nu.
+ * No direct reference counterpart (the reference always gathers inside the Python
+ * calculator); exposed for callers that keep the gathered data resident.
+ * ------------------------------------------------------------------------------- */
+size_t tb_sweep_workspace_size(int32_t k, int32_t batch);
+int32_t tb_sweep_minors_batch(const tb_complex128 *Vt, int32_t k, int32_t ldv,
+                              int64_t item_stride, int32_t batch, tb_complex128 *minors,
+                              void *workspace, size_t workspace_bytes, void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * Permanent and its gradient d perm(A) / d A[i][j] = perm(A without row i, column j)
+ * for a batch of n x n matrices. No reference counterpart (the reference has no
+ * differentiation); each gradient row reuses the C&C submatrices sweep with that row
+ * deleted, recomputing instead of storing per-subset sums.
+ *   perm_out   batch permanents (Laplace expansion of the gradient along row 0)
+ *   grad_out   batch x n x n
+ * ------------------------------------------------------------------------------- */
+size_t tb_perm_grad_workspace_size(int32_t n, int32_t batch);
+int32_t tb_perm_grad_batch(const tb_complex128 *A, int32_t n, int32_t batch,
+                           int64_t matrix_stride, tb_complex128 *perm_out,
+                           tb_complex128 *grad_out, void *workspace, size_t workspace_bytes,
+                           void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * Batched generalized Clifford & Clifford (GCC-B) sampler.
+ * Replaces GeneralizedCliffordsBSimulationStrategy.simulate / _fill_r_sample
+ * (theboss/simulation_strategies/generalized_cliffords_b_simulation_strategy.py:41-110)
+ * and its uniform-loss variant
+ * (generalized_cliffords_b_uniform_losses_simulation_strategy.py:50-121) for many
+ * samples at once: every step k of every sample runs the C&C sweep, the Laplace
+ * expansion over all m output modes and the inverse-CDF draw on the device.
+ *
+ *   U             m x m interferometer (row-major, ldu)
+ *   particles     n input particles as mode indices (mode assignment of the input state,
+ *                 boson_sampling_utilities.py:60-78)
+ *   seed          Philox-4x32-10 key for the device RNG
+ *   samples       number of samples S
+ *   transmissivity  eta in [0,1]; 1 = lossless GCC-B
+ *   forced_order  optional S x n particle orders (NULL = device RNG)
+ *   forced_uniforms optional S x n uniforms for the output draws (NULL = device RNG)
+ *   forced_steps  optional S step counts (NULL = n if eta == 1 else binomial draw)
+ *   out_samples   S x m int32 output occupations
+ *   trace_minors  optional S x n x n complex minors trace (step k row k-1), NULL = off
+ * The sampler object owns no device memory: plan with tb_gccb_plan_size, pass a
+ * workspace of that many bytes.
+ * ------------------------------------------------------------------------------- */
+typedef struct {
+  int32_t m, n, samples;
+  int32_t chunk_bits[TB_SWEEP_MAX_K + 1]; /* per step k: Gray bits per lane chunk (0 = auto) */
+} tb_gccb_config;
+
+size_t tb_gccb_workspace_size(const tb_gccb_config *cfg);
+int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128 *U, int32_t ldu,
+                       const int32_t *particles, uint64_t seed, double transmissivity,
+                       const int32_t *forced_order, const double *forced_uniforms,
+                       const int32_t *forced_steps, int32_t *out_samples,
+                       tb_complex128 *trace_minors, void *workspace, size_t workspace_bytes,
+                       void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* THEBOSS_AMD_H */

@@ -1,0 +1,325 @@
+"""Parity of the gfx950 kernels with the oracle (run on an MI355X).
+
+Bars: FP64 paths at relative error <= 1e-10 against the oracle; integer outputs of the
+sampler bit-exact against the oracle sampler driven by the same forced draws.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10  # FP64 relative-error bar for permanents
+
+
+@pytest.fixture(scope="module")
+def T():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from theboss_amd import _lib
+
+    _lib.load()
+    return torch
+
+
+@pytest.fixture(scope="module")
+def ops(T):
+    from theboss_amd import ops as o
+
+    return o
+
+
+def dev_c(T, a):
+    return T.from_numpy(np.ascontiguousarray(a, dtype=np.complex128)).cuda()
+
+
+def dev_i(T, a):
+    return T.from_numpy(np.ascontiguousarray(a, dtype=np.int32)).cuda()
+
+
+def rel(a, b):
+    return np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(np.asarray(b))), 1e-300)
+
+
+# ------------------------------------------------------------------------------ Glynn
+@pytest.mark.parametrize("n", list(range(0, 16)))
+def test_glynn_fp64_small_n_vs_oracle(T, ops, oracle, n):
+    B = 5
+    As = np.stack([oracle.haar_unitary(max(2 * n, 1), 10 * n + b)[:n, :n] for b in range(B)]) \
+        if n else np.zeros((B, 0, 0), complex)
+    got = ops.glynn_batch(dev_c(T, As)).cpu().numpy()
+    if n == 0:
+        assert np.all(got == 1)
+        return
+    ref = np.array([oracle.perm_glynn_matrix(A, 1) for A in As])
+    for g, r in zip(got, ref):
+        assert abs(g - r) <= REL * abs(r), (n, g, r)
+
+
+@pytest.mark.parametrize("n", [16, 20, 22])
+def test_glynn_fp64_medium_vs_oracle(T, ops, oracle, n):
+    As = np.stack([oracle.haar_unitary(2 * n, 1000 + n + b)[:n, :n] for b in range(3)])
+    got = ops.glynn_batch(dev_c(T, As)).cpu().numpy()
+    ref = oracle.perm_glynn_batch(As, 1)
+    for g, r in zip(got, ref):
+        assert abs(g - r) <= REL * abs(r), (n, g, r)
+
+
+@pytest.mark.parametrize("n", [24, 28])
+def test_glynn_fp64_closed_forms(T, ops, oracle, n):
+    import math
+
+    rng = np.random.default_rng(n)
+    u = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    mats = np.stack([np.outer(u, v), np.ones((n, n)), np.ones((n, n)) - np.eye(n)])
+    got = ops.glynn_batch(dev_c(T, mats)).cpu().numpy()
+    expect = [math.factorial(n) * np.prod(u) * np.prod(v), math.factorial(n),
+              oracle.derangements(n)]
+    for g, e in zip(got, expect):
+        assert abs(g - e) <= REL * abs(e), (n, g, e)
+
+
+def test_glynn_fp32_colsum_error(T, ops, oracle):
+    from theboss_amd import _lib
+
+    for n in (8, 12, 16):
+        As = np.stack([oracle.haar_unitary(2 * n, 77 + n + b)[:n, :n] for b in range(4)])
+        got = ops.glynn_batch(dev_c(T, As), _lib.TB_PREC_FP32_COLSUM).cpu().numpy()
+        ref = np.array([oracle.perm_glynn_matrix(A, 1) for A in As])
+        err = np.max(np.abs(got - ref) / np.abs(ref))
+        assert err < 1e-2, (n, err)
+
+
+def test_glynn_deterministic(T, ops, oracle):
+    A = oracle.haar_unitary(40, 5)[:20, :20]
+    a = ops.glynn_batch(dev_c(T, A[None])).cpu().numpy()
+    b = ops.glynn_batch(dev_c(T, A[None])).cpu().numpy()
+    assert np.array_equal(a, b)
+
+
+# ------------------------------------------------------------------------ Guan kernels
+PATTERNS = [
+    ([1, 1, 1, 1], [1, 1, 1, 1]),
+    ([1, 1, 1, 0], [1, 0, 1, 1]),
+    ([2, 1, 0, 0], [0, 1, 1, 1]),
+    ([1, 1, 1, 0], [2, 1, 0, 0]),
+    ([2, 1, 1, 0], [1, 1, 0, 2]),
+    ([0, 0, 4, 0], [1, 1, 0, 2]),
+    ([1, 1, 0, 2], [0, 1, 3, 0]),
+    ([0, 2, 0, 2], [0, 0, 4, 0]),
+    ([0, 0, 0, 0], [0, 0, 0, 0]),
+]
+
+
+@pytest.mark.parametrize("ctype", ["CLASSIC", "GLYNN", "CHIN_HUH", "RYSER"])
+def test_calculators_reference_patterns(T, oracle, ctype):
+    from theboss_amd.permanent_calculators import BSPermanentCalculatorFactory, PermanentCalculatorType
+
+    U = oracle.haar_unitary(4, 11)
+    calc = BSPermanentCalculatorFactory(U, None, None, PermanentCalculatorType[ctype]).generate_calculator()
+    for s, t in PATTERNS:
+        calc.input_state, calc.output_state = s, t
+        got = calc.compute_permanent()
+        expect = oracle.perm_classic(U, s, t)
+        assert abs(got - expect) <= 1e-12 * max(1.0, abs(expect)), (ctype, s, t, got, expect)
+
+
+def test_guan_random_multiplicities_batch(T, oracle):
+    from theboss_amd.permanent_calculators import GPUChinHuhPermanentCalculator, GPURyserPermanentCalculator
+
+    rng = np.random.default_rng(5)
+    m = 7
+    U = oracle.haar_unitary(m, 99)
+    ins, outs = [], []
+    for _ in range(40):
+        n = int(rng.integers(1, 8))
+        ins.append(np.bincount(rng.integers(0, m, n), minlength=m))
+        outs.append(np.bincount(rng.integers(0, m, n), minlength=m))
+    ref = np.array([oracle.perm_classic(U, s, t) for s, t in zip(ins, outs)])
+    for cls in (GPURyserPermanentCalculator, GPUChinHuhPermanentCalculator):
+        got = cls(U).compute_permanent_batch(ins, outs)
+        assert np.max(np.abs(got - ref)) <= 1e-12, cls.__name__
+
+
+def test_shape_mismatch_raises_attribute_error(T, oracle):
+    from theboss_amd.permanent_calculators import GPURyserPermanentCalculator
+
+    calc = GPURyserPermanentCalculator(oracle.haar_unitary(4, 1), [1, 1, 0, 0], [1, 1, 0])
+    with pytest.raises(AttributeError):
+        calc.compute_permanent()
+
+
+# ---------------------------------------------------------------- C&C submatrices
+@pytest.mark.parametrize("s,t", [
+    ([1, 1, 1, 1], [0, 1, 1, 1]),
+    ([1, 3, 0, 0], [0, 1, 1, 1]),
+    ([1, 0, 0, 0], [0, 0, 0, 0]),
+])
+def test_submatrices_reference_cases(T, oracle, s, t):
+    from theboss_amd.permanent_calculators import GPUBSCCRyserSubmatricesPermanentCalculator
+
+    U = oracle.haar_unitary(4, 21)
+    got = np.asarray(GPUBSCCRyserSubmatricesPermanentCalculator(U, np.array(s, float), t).compute_permanents())
+    ref = oracle.cc_submatrices(U, s, t)
+    assert got.shape == (4,)
+    assert np.max(np.abs(got - ref)) <= 1e-12
+
+
+@pytest.mark.parametrize("k", [2, 3, 5, 8, 12, 16, 20])
+def test_submatrices_binary_sweep_vs_oracle(T, oracle, k):
+    from theboss_amd.permanent_calculators import GPUBSCCRyserSubmatricesPermanentCalculator
+
+    rng = np.random.default_rng(k)
+    m = 3 * k
+    U = oracle.haar_unitary(m, 500 + k)
+    s = np.zeros(m, int)
+    s[rng.choice(m, k, replace=False)] = 1
+    t = np.bincount(rng.integers(0, m, k - 1), minlength=m)  # outputs may collide
+    got = np.asarray(GPUBSCCRyserSubmatricesPermanentCalculator(U, s, t).compute_permanents())
+    ref = oracle.cc_submatrices(U, s, t)
+    assert rel(got, ref) <= REL, (k, rel(got, ref))
+
+
+def test_submatrices_collision_inputs_vs_oracle(T, oracle):
+    from theboss_amd.permanent_calculators import GPUBSCCRyserSubmatricesPermanentCalculator
+
+    rng = np.random.default_rng(3)
+    for trial in range(10):
+        m = 6
+        k = int(rng.integers(2, 8))
+        U = oracle.haar_unitary(m, 700 + trial)
+        s = np.bincount(rng.integers(0, 3, k), minlength=m)
+        t = np.bincount(rng.integers(0, m, k - 1), minlength=m)
+        got = np.asarray(GPUBSCCRyserSubmatricesPermanentCalculator(U, s, t).compute_permanents())
+        ref = oracle.cc_submatrices(U, s, t)
+        assert np.max(np.abs(got - ref)) <= 1e-12, (s, t)
+
+
+def test_sweep_k20_laplace_identity(T, ops, oracle):
+    """Size-independent property at k = 20: sum_nu U[o][nu] minor_nu = perm(U[t+e_o, s])."""
+    k = 20
+    U = oracle.haar_unitary(40, 4)
+    A = U[:k, :k]  # rows = outputs 0..19, columns = inputs 0..19
+    Vt = A[: k - 1, :].T.copy()  # particle nu -> column of the first k-1 output rows
+    minors = ops.sweep_minors_batch(dev_c(T, Vt[None]), k).cpu().numpy()[0]
+    lap = np.sum(A[k - 1, :] * minors)
+    ref = oracle.perm_glynn_matrix(A, 1)
+    assert abs(lap - ref) <= REL * abs(ref)
+
+
+# ------------------------------------------------------------------------- Gradient
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 7])
+def test_grad_vs_oracle_minors(T, ops, oracle, n):
+    As = np.stack([oracle.haar_unitary(2 * n, 40 + n + b)[:n, :n] for b in range(3)])
+    perm, G = ops.perm_grad_batch(dev_c(T, As))
+    perm, G = perm.cpu().numpy(), G.cpu().numpy()
+    for b in range(3):
+        assert abs(perm[b] - oracle.perm_classic_matrix(As[b])) <= 1e-12
+        assert np.max(np.abs(G[b] - oracle.perm_minors(As[b]))) <= 1e-12
+
+
+def test_grad_n12_laplace_and_oracle(T, ops, oracle):
+    n = 12
+    A = oracle.haar_unitary(24, 8)[:n, :n]
+    perm, G = ops.perm_grad_batch(dev_c(T, A[None]))
+    G = G.cpu().numpy()[0]
+    ref = oracle.perm_minors(A)
+    assert rel(G, ref) <= REL
+    p = oracle.perm_glynn_matrix(A, 1)
+    for i in range(n):
+        assert abs(np.sum(A[i] * G[i]) - p) <= REL * abs(p)
+
+
+# -------------------------------------------------------------------------- Sampler
+def _forced(rng, S, n):
+    order = np.stack([rng.permutation(n) for _ in range(S)]).astype(np.int32)
+    unif = rng.random((S, n))
+    return order, unif
+
+
+@pytest.mark.parametrize("m,n,S", [(6, 3, 64), (10, 5, 200), (24, 8, 64)])
+def test_batched_sampler_forced_draws_match_oracle(T, ops, oracle, m, n, S):
+    rng = np.random.default_rng(m + n)
+    U = oracle.haar_unitary(m, m * n)
+    particles = np.arange(n, dtype=np.int32)
+    order, unif = _forced(rng, S, n)
+    out, trace = ops.gccb_sample(dev_c(T, U), dev_i(T, particles), S, 1, 1.0, dev_i(T, order),
+                                 T.from_numpy(unif).cuda(), None, trace=True)
+    expect = oracle.gccb_sample_batch(U, order, unif)
+    assert np.array_equal(out.cpu().numpy(), expect)
+    # Per-step minors against the oracle's C&C sweep on the same prefix.
+    tr = trace.cpu().numpy()
+    for s in range(min(S, 8)):
+        cur = np.zeros(m, int)
+        r = np.zeros(m, int)
+        hist = []
+        for k in range(1, n + 1):
+            cur[particles[order[s, k - 1]]] += 1
+            ref = oracle.cc_submatrices(U, cur, r)
+            got = tr[s, k - 1, :k]
+            want = np.array([ref[particles[order[s, nu]]] for nu in range(k)])
+            assert rel(got, want) <= REL
+            pmf, _ = oracle.gccb_pmf(U, cur, r)
+            r[int(np.searchsorted(np.cumsum(pmf), unif[s, k - 1], side="right"))] += 1
+            hist.append(k)
+
+
+def test_batched_sampler_uniform_loss_forced_steps(T, ops, oracle):
+    m, n, S = 12, 6, 128
+    rng = np.random.default_rng(1)
+    U = oracle.haar_unitary(m, 12)
+    particles = np.arange(n, dtype=np.int32)
+    order, unif = _forced(rng, S, n)
+    steps = rng.integers(0, n + 1, S).astype(np.int32)
+    out, _ = ops.gccb_sample(dev_c(T, U), dev_i(T, particles), S, 1, 0.5, dev_i(T, order),
+                             T.from_numpy(unif).cuda(), dev_i(T, steps))
+    expect = oracle.gccb_sample_batch(U, order, unif, steps)
+    assert np.array_equal(out.cpu().numpy(), expect)
+    assert np.array_equal(out.cpu().numpy().sum(1), steps)
+
+
+def test_batched_sampler_philox_statistics(T, ops, oracle):
+    """Device RNG path: samples follow the exact distribution (TVD bound of the
+    reference's tests, quantum_computations_utilities.py:95-127)."""
+    m, n, S = 4, 2, 20000
+    U = oracle.haar_unitary(m, 31)
+    particles = np.array([0, 1], dtype=np.int32)
+    out, _ = ops.gccb_sample(dev_c(T, U), dev_i(T, particles), S, 1234)
+    out = out.cpu().numpy()
+    states = oracle.generate_possible_states(n, m)
+    freq = np.zeros(len(states))
+    index = {s: i for i, s in enumerate(states)}
+    for row in out:
+        freq[index[tuple(int(x) for x in row)]] += 1
+    freq /= S
+    exact = np.array([oracle.fixed_losses_probability(U, [1, 1, 0, 0], st, oracle.perm_ryser)
+                      for st in states])
+    assert abs(exact.sum() - 1) < 1e-12
+    tvd = 0.5 * np.abs(freq - exact).sum()
+    assert tvd < oracle.tvd_error_bound(len(states), S, 1e-4)
+
+
+def test_plain_strategy_equals_batched_with_same_draws(T, ops, oracle):
+    from theboss_amd.permanent_calculators import GPURyserPermanentCalculator
+    from theboss_amd.simulation_strategies import GeneralizedCliffordsBSimulationStrategy
+
+    m, n, S = 16, 6, 12
+    rng = np.random.default_rng(2)
+    U = oracle.haar_unitary(m, 77)
+    inp = [1] * n + [0] * (m - n)
+    order, unif = _forced(rng, S, n)
+    plain = GeneralizedCliffordsBSimulationStrategy(GPURyserPermanentCalculator(U)).simulate_with_draws(
+        inp, order, unif)
+    out, _ = ops.gccb_sample(dev_c(T, U), dev_i(T, np.arange(n)), S, 1, 1.0, dev_i(T, order),
+                             T.from_numpy(unif).cuda(), None)
+    assert np.array_equal(out.cpu().numpy(), plain)
+
+
+def test_errors_surface_as_exceptions(T, ops):
+    from theboss_amd._lib import TbError
+
+    A = T.zeros((1, 49, 49), dtype=T.complex128, device="cuda")
+    with pytest.raises(TbError):
+        ops.glynn_batch(A)

@@ -1,0 +1,72 @@
+// FP64 Glynn: instantiates the launch templates for n = 2..TB_GLYNN_MAX_N and holds the
+// C entry points.
+#include "glynn_impl.hpp"
+
+namespace tbgl {
+int32_t glynn_dispatch_f64(int n, const tb::cpx<double> *A, int64_t mstride, int batch,
+                           tb::cd *out, tb::cd *partial, hipStream_t s) {
+  return dispatch_glynn<double>(n, A, mstride, batch, out, partial, s);
+}
+}  // namespace tbgl
+
+namespace {
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t tbgl_partial_bytes(int n, int batch) {
+  if (n < tbgl::kWideMinN) return 0;
+  return align256((size_t)batch * tbgl::glynn_plan(n).wpm * sizeof(tb::cd));
+}
+}  // namespace
+
+extern "C" size_t tb_glynn_workspace_size(int32_t n, int32_t batch, int32_t precision) {
+  if (n <= 1 || batch <= 0 || n > TB_GLYNN_MAX_N) return 0;
+  size_t bytes = tbgl_partial_bytes(n, batch);
+  if (precision == TB_PREC_FP32_COLSUM) bytes += align256((size_t)batch * n * n * sizeof(tb::cpx<float>));
+  return bytes;
+}
+
+extern "C" int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t batch,
+                                  int64_t matrix_stride, int32_t precision, tb_complex128 *out,
+                                  void *workspace, size_t workspace_bytes, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (batch < 0) return tbh::fail(TB_EINVAL, "glynn: batch=%d < 0", batch);
+  if (batch == 0) return TB_OK;
+  if (!out) return tbh::fail(TB_EINVAL, "glynn: out is NULL");
+  if (n < 0) return tbh::fail(TB_EINVAL, "glynn: n=%d < 0", n);
+  if (n > TB_GLYNN_MAX_N)
+    return tbh::fail(TB_ERANGE, "glynn: n=%d exceeds TB_GLYNN_MAX_N=%d", n, TB_GLYNN_MAX_N);
+  if (precision != TB_PREC_FP64 && precision != TB_PREC_FP32_COLSUM)
+    return tbh::fail(TB_EINVAL, "glynn: unknown precision %d", precision);
+  tb::cd *o = (tb::cd *)out;
+  if (n == 0) {
+    hipLaunchKernelGGL(tb::fill_one_kernel, dim3((batch + 255) / 256), dim3(256), 0, s, o, batch);
+    return tbh::check_hip(hipGetLastError(), "fill_one_kernel");
+  }
+  if (!A) return tbh::fail(TB_EINVAL, "glynn: A is NULL");
+  if (matrix_stride < (int64_t)n * n)
+    return tbh::fail(TB_EINVAL, "glynn: matrix_stride=%lld < n*n=%d", (long long)matrix_stride,
+                     n * n);
+  const size_t need = tb_glynn_workspace_size(n, batch, precision);
+  if (need > 0 && (workspace == nullptr || workspace_bytes < need))
+    return tbh::fail(TB_EWORKSPACE, "glynn: workspace %zu bytes < required %zu", workspace_bytes,
+                     need);
+  if (n == 1) {
+    hipLaunchKernelGGL(tb::copy_scalar_kernel, dim3((batch + 255) / 256), dim3(256), 0, s,
+                       (const tb::cd *)A, matrix_stride, batch, o);
+    return tbh::check_hip(hipGetLastError(), "copy_scalar_kernel");
+  }
+  char *ws = (char *)workspace;
+  tb::cd *partial = (tb::cd *)ws;
+  if (precision == TB_PREC_FP64)
+    return tbgl::glynn_dispatch_f64(n, (const tb::cpx<double> *)A, matrix_stride, batch, o,
+                                    partial, s);
+  // FP32 column sums: convert once into the tail of the workspace.
+  float *A32 = (float *)(ws + tbgl_partial_bytes(n, batch));
+  const int64_t total = (int64_t)batch * n * n;
+  hipLaunchKernelGGL(tb::to_fp32_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                     (const tb::cd *)A, matrix_stride, n, batch, A32);
+  int32_t rc = tbh::check_hip(hipGetLastError(), "to_fp32_kernel");
+  if (rc) return rc;
+  return tbgl::glynn_dispatch_f32(n, (const tb::cpx<float> *)A32, (int64_t)n * n, batch, o,
+                                    partial, s);
+}

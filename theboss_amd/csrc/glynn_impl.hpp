@@ -1,0 +1,395 @@
+#pragma once
+// Glynn permanents with a reflected Gray code, gfx950 (kernels and launch templates;
+// instantiated by glynn_f64.hip and glynn_f32.hip).
+//
+// Restates GlynnGrayPermanentCalculator.compute_permanent
+// (theboss/boson_sampling_utilities/permanent_calculators/glynn_gray_permanent_calculator.py:41-84):
+//   perm(M) = 2^-(n-1) * sum_{delta in {+-1}^{n-1} x {+1}} (prod_i delta_i) prod_j sum_i delta_i M[i][j]
+// enumerated in Gray order (one delta flip per term).
+//
+// Work split: the 2^(n-1) Gray ranks are cut into aligned chunks of 2^C ranks. One lane
+// owns one chunk: it seeds its column sums at the chunk start in O(n^2) and then walks
+// the chunk with O(n) updates per term. Because chunks are aligned, the row flipped at
+// local step l is ctz(l) for every lane, so in WIDE mode (all 64 lanes of a wave on one
+// matrix) the flipped row is wave-uniform and is read with scalar loads (SGPR operands)
+// instead of 64 replicated vector loads.
+//
+// Half column sums h_j = s_j / 2 are carried so a flip is a plain add/sub of M[i][j];
+// perm = 2 * sum_t sign(t) prod_j h_j(t). sign(t) = (-1)^(popcount gray(t)) = (-1)^(t & 1).
+#include <algorithm>
+
+#include "tb_common.hpp"
+
+namespace tb {
+
+// Keeps the scheduler from hoisting the next Gray step's row reads above the current
+// step (which otherwise multiplies live registers by the unroll factor and spills).
+// Occupancy target for the wide kernels: 2 waves per SIMD (<= 256 VGPRs).
+#ifndef TB_WIDE_ATTR
+#define TB_WIDE_ATTR __attribute__((amdgpu_waves_per_eu(2)))
+#endif
+
+#ifndef TB_STEP_FENCE
+#define TB_STEP_FENCE() __builtin_amdgcn_sched_barrier(0)
+#endif
+
+template <typename F>
+struct cpx {
+  F re, im;
+};
+
+template <typename F>
+__device__ __forceinline__ cpx<F> cmulT(cpx<F> a, cpx<F> b) {
+  cpx<F> r;
+  r.re = fma(a.re, b.re, -(a.im * b.im));
+  r.im = fma(a.re, b.im, a.im * b.re);
+  return r;
+}
+
+template <typename F, int LO, int HI>
+struct TreeProdT {
+  static __device__ __forceinline__ cpx<F> run(const F *re, const F *im) {
+    constexpr int MID = (LO + HI) / 2;
+    return cmulT<F>(TreeProdT<F, LO, MID>::run(re, im), TreeProdT<F, MID, HI>::run(re, im));
+  }
+};
+template <typename F, int LO>
+struct TreeProdT<F, LO, LO + 1> {
+  static __device__ __forceinline__ cpx<F> run(const F *re, const F *im) { return {re[LO], im[LO]}; }
+};
+
+// h += row (add) or h -= row; `row` is a wave-uniform LDS row (broadcast reads) or a
+// per-lane global row (NARROW mode). The row is consumed in groups of G complex values
+// with the reads of group g+1 issued before the adds of group g, and a scheduling
+// barrier per group, so at most 2*G loaded values are live at once.
+#ifndef TB_FLIP_GROUP
+#define TB_FLIP_GROUP 4
+#endif
+
+// MODE 1: h += row, MODE -1: h -= row, MODE 0: h += sg * row (runtime direction, one
+// FMA per component with sg = +-1 held per lane).
+template <int N, int MODE, typename F, typename P>
+__device__ __forceinline__ void flip_mem(F (&hr)[N], F (&hi)[N], P row, F sg) {
+  constexpr int G = TB_FLIP_GROUP;
+  constexpr int NG = (N + G - 1) / G;
+  cpx<F> cur[G], nxt[G];
+#pragma unroll
+  for (int e = 0; e < G; ++e)
+    if (e < N) cur[e] = row[e];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    if (g + 1 < NG) {
+#pragma unroll
+      for (int e = 0; e < G; ++e)
+        if ((g + 1) * G + e < N) nxt[e] = row[(g + 1) * G + e];
+    }
+#pragma unroll
+    for (int e = 0; e < G; ++e) {
+      const int j = g * G + e;
+      if (j < N) {
+        if constexpr (MODE == 1) {
+          hr[j] += cur[e].re;
+          hi[j] += cur[e].im;
+        } else if constexpr (MODE == -1) {
+          hr[j] -= cur[e].re;
+          hi[j] -= cur[e].im;
+        } else {
+          hr[j] = fma(sg, cur[e].re, hr[j]);
+          hi[j] = fma(sg, cur[e].im, hi[j]);
+        }
+      }
+    }
+    TB_STEP_FENCE();
+#pragma unroll
+    for (int e = 0; e < G; ++e) cur[e] = nxt[e];
+  }
+}
+
+template <int N, int MODE, typename F>
+__device__ __forceinline__ void flip_reg(F (&hr)[N], F (&hi)[N], const F (&rr)[N], const F (&ri)[N],
+                                         F sg) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    if constexpr (MODE == 1) {
+      hr[j] += rr[j];
+      hi[j] += ri[j];
+    } else if constexpr (MODE == -1) {
+      hr[j] -= rr[j];
+      hi[j] -= ri[j];
+    } else {
+      hr[j] = fma(sg, rr[j], hr[j]);
+      hi[j] = fma(sg, ri[j], hi[j]);
+    }
+  }
+}
+
+// Rows 0..R0-1 (the most frequently flipped: row i flips on 2^-(i+1) of all steps) are
+// held in VGPRs; the rest are read from memory at each flip.
+template <int N, int R0, typename F>
+struct RegRows {
+  F re[R0 > 0 ? R0 : 1][N], im[R0 > 0 ? R0 : 1][N];
+};
+
+template <int N, int R0, int MODE, typename F, typename P>
+__device__ __forceinline__ void flip_row(F (&hr)[N], F (&hi)[N], const RegRows<N, R0, F> &rr,
+                                         P M, int i, F sg) {
+  if constexpr (R0 > 0) {
+    if (i < R0) {
+      // i is a compile-time constant at every static flip site after unrolling.
+#pragma unroll
+      for (int r = 0; r < R0; ++r)
+        if (r == i) flip_reg<N, MODE, F>(hr, hi, rr.re[r], rr.im[r], sg);
+      return;
+    }
+  }
+  flip_mem<N, MODE, F>(hr, hi, M + (size_t)i * N, sg);
+}
+
+// One lane walks `iters` chunks q = q0 + it*qstride of 2^C Gray ranks each; M is the
+// n x n matrix as complex rows. `topbit_fixed` means C == N-1 (the whole enumeration in
+// one chunk), so the Gray bit above the chunk is 0.
+template <int N, int U, int R0, typename F, typename P>
+__device__ __forceinline__ void glynn_lane(P M, int C, uint64_t q0, uint64_t qstride, int iters,
+                                           double &acc_re, double &acc_im) {
+  constexpr int NB = N - 1;  // Gray bits
+  const bool topbit_fixed = (C == NB);
+  RegRows<N, R0, F> rr;
+  if constexpr (R0 > 0) {
+#pragma unroll
+    for (int r = 0; r < R0; ++r)
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const cpx<F> v = M[(size_t)r * N + j];
+        rr.re[r][j] = v.re;
+        rr.im[r][j] = v.im;
+      }
+  }
+  for (int it = 0; it < iters; ++it) {
+    const uint64_t q = q0 + (uint64_t)it * qstride;
+    const uint64_t t0 = q << C;
+    const uint64_t g0 = t0 ^ (t0 >> 1);
+    F hr[N], hi[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const cpx<F> v = M[(size_t)(N - 1) * N + j];
+      hr[j] = v.re;
+      hi[j] = v.im;
+    }
+#pragma unroll 1
+    for (int i = 0; i < NB; ++i) {
+      const F sg = ((g0 >> i) & 1ull) ? F(-1) : F(1);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const cpx<F> v = M[(size_t)i * N + j];
+        hr[j] = fma(sg, v.re, hr[j]);
+        hi[j] = fma(sg, v.im, hi[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      hr[j] *= F(0.5);
+      hi[j] *= F(0.5);
+    }
+    const bool q_odd = (q & 1ull) != 0;
+    double cre, cim;
+    {
+      cpx<F> p = TreeProdT<F, 0, N>::run(hr, hi);
+      cre = (double)p.re;
+      cim = (double)p.im;
+    }
+    const int nblocks = 1 << (C - U);
+    for (int blk = 0; blk < nblocks; ++blk) {
+      if (blk > 0) {
+        const int i = U + __builtin_ctz(blk);
+        bool add;
+        if (i + 1 < C)
+          add = ((blk >> (i + 1 - U)) & 1) != 0;
+        else
+          add = topbit_fixed ? false : q_odd;
+        flip_mem<N, 0, F>(hr, hi, M + (size_t)i * N, add ? F(1) : F(-1));
+        cpx<F> p = TreeProdT<F, 0, N>::run(hr, hi);
+        cre += (double)p.re;
+        cim += (double)p.im;
+        TB_STEP_FENCE();
+      }
+#pragma unroll
+      for (int j = 1; j < (1 << U); ++j) {
+        const int i = __builtin_ctz(j);
+        if (i + 1 < U) {
+          // Direction fixed by the local bit i+1 of j.
+          if ((j >> (i + 1)) & 1)
+            flip_row<N, R0, 1, F>(hr, hi, rr, M, i, F(1));
+          else
+            flip_row<N, R0, -1, F>(hr, hi, rr, M, i, F(-1));
+        } else {
+          bool add;
+          if (U < C)
+            add = (blk & 1) != 0;
+          else
+            add = topbit_fixed ? false : q_odd;
+          flip_row<N, R0, 0, F>(hr, hi, rr, M, i, add ? F(1) : F(-1));
+        }
+        cpx<F> p = TreeProdT<F, 0, N>::run(hr, hi);
+        if (j & 1) {
+          cre -= (double)p.re;
+          cim -= (double)p.im;
+        } else {
+          cre += (double)p.re;
+          cim += (double)p.im;
+        }
+        TB_STEP_FENCE();
+      }
+    }
+    acc_re += cre;
+    acc_im += cim;
+  }
+}
+
+// WIDE: the 4 waves of a workgroup share one matrix, staged once into LDS; every flip is
+// a broadcast LDS read. Grid: batch * (wpm/4) workgroups; wave wq of matrix b covers
+// chunks q = (wq + it*wpm)*64 + lane. Strides count complex elements.
+template <int N, int U, int R0, typename F>
+__global__ __launch_bounds__(256) TB_WIDE_ATTR void glynn_wide_kernel(const cpx<F> *__restrict__ A,
+                                                         int64_t mstride, int C, int wpm, int iters,
+                                                         cd *__restrict__ partial) {
+  __shared__ cpx<F> Ms[N * N];
+  const int lane = threadIdx.x & 63;
+  const int wgs_per_matrix = wpm >> 2;
+  const int b = blockIdx.x / wgs_per_matrix;
+  const int wq = (blockIdx.x - b * wgs_per_matrix) * 4 + (threadIdx.x >> 6);
+  const cpx<F> *src = A + (size_t)b * mstride;
+  for (int e = threadIdx.x; e < N * N; e += 256) Ms[e] = src[e];
+  __syncthreads();
+  double are = 0.0, aim = 0.0;
+  glynn_lane<N, U, R0, F>((const cpx<F> *)Ms, C, (uint64_t)wq * 64 + lane, (uint64_t)wpm * 64,
+                          iters, are, aim);
+  are = wave_sum(are);
+  aim = wave_sum(aim);
+  if (lane == 0) partial[(size_t)b * wpm + wq] = {are, aim};
+}
+
+// NARROW: one lane per matrix, whole enumeration in the lane (small n), global rows.
+template <int N, int U, typename F>
+__global__ __launch_bounds__(256) void glynn_narrow_kernel(const cpx<F> *__restrict__ A,
+                                                           int64_t mstride, int batch,
+                                                           cd *__restrict__ out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  const cpx<F> *M = A + (size_t)b * mstride;
+  double are = 0.0, aim = 0.0;
+  glynn_lane<N, U, 0, F>(M, N - 1, 0, 0, 1, are, aim);
+  out[b] = {2.0 * are, 2.0 * aim};
+}
+
+namespace {  // internal linkage: each translation unit keeps its own copy
+
+// Sum the wpm wave partials of each matrix in a fixed order: deterministic results.
+__global__ __launch_bounds__(256) void glynn_finish_kernel(const cd *__restrict__ partial, int wpm,
+                                                           int batch, cd *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (b >= batch) return;
+  double re = 0.0, im = 0.0;
+  for (int w = lane; w < wpm; w += 64) {
+    re += partial[(size_t)b * wpm + w].re;
+    im += partial[(size_t)b * wpm + w].im;
+  }
+  re = wave_sum(re);
+  im = wave_sum(im);
+  if (lane == 0) out[b] = {2.0 * re, 2.0 * im};
+}
+
+__global__ void fill_one_kernel(cd *out, int batch) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < batch) out[b] = {1.0, 0.0};
+}
+
+__global__ void copy_scalar_kernel(const cd *A, int64_t mstride, int batch, cd *out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < batch) out[b] = A[(size_t)b * mstride];
+}
+
+// FP32 copy of the matrices (row store for the FP32 column-sum variant).
+__global__ void to_fp32_kernel(const cd *A, int64_t mstride, int n, int batch, float *dst) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)n * n;
+  if (idx >= per * batch) return;
+  const int64_t b = idx / per, e = idx - b * per;
+  const cd v = A[b * mstride + e];
+  dst[2 * idx] = (float)v.re;
+  dst[2 * idx + 1] = (float)v.im;
+}
+
+}  // namespace
+
+}  // namespace tb
+
+// ------------------------------------------------------------------------------------
+// Host side
+// ------------------------------------------------------------------------------------
+namespace tbgl {
+
+#ifndef TB_GLYNN_R0
+#define TB_GLYNN_R0 0
+#endif
+
+constexpr int kWideMinN = 11;  // below this a lane owns a whole matrix
+constexpr int kMaxC = 12;      // Gray bits per lane chunk
+constexpr int kMaxWpmBits = 13;  // waves per matrix before lanes iterate over chunks
+
+struct GlynnPlan {
+  int C, wpm, iters;
+};
+
+// WIDE plan: chunks of 2^C ranks, at least 4 waves (one workgroup) per matrix.
+inline GlynnPlan glynn_plan(int n) {
+  GlynnPlan p{};
+  const int nb = n - 1;
+  p.C = std::min(kMaxC, nb - 8);
+  const int wave_bits = nb - p.C - 6;  // log2(waves needed at one chunk per lane), >= 2
+  const int wpm_bits = std::min(wave_bits, kMaxWpmBits);
+  p.wpm = 1 << wpm_bits;
+  p.iters = 1 << (wave_bits - wpm_bits);
+  return p;
+}
+
+template <int N, typename F>
+inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, int batch, tb::cd *out,
+                       tb::cd *partial, hipStream_t s) {
+  if constexpr (N < kWideMinN) {
+    constexpr int U = (N - 1) < 2 ? (N - 1) : 2;
+    hipLaunchKernelGGL((tb::glynn_narrow_kernel<N, U, F>), dim3((batch + 255) / 256), dim3(256), 0,
+                       s, A, mstride, batch, out);
+    (void)partial;
+    return tbh::check_hip(hipGetLastError(), "glynn_narrow_kernel");
+  } else {
+    const GlynnPlan p = glynn_plan(N);
+    constexpr int U = 2;
+    const unsigned blocks = (unsigned)batch * (unsigned)(p.wpm / 4);
+    hipLaunchKernelGGL((tb::glynn_wide_kernel<N, U, TB_GLYNN_R0, F>), dim3(blocks), dim3(256), 0, s,
+                       A, mstride, p.C, p.wpm, p.iters, partial);
+    int32_t rc = tbh::check_hip(hipGetLastError(), "glynn_wide_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL(tb::glynn_finish_kernel, dim3((batch + 3) / 4), dim3(256), 0, s, partial,
+                       p.wpm, batch, out);
+    return tbh::check_hip(hipGetLastError(), "glynn_finish_kernel");
+  }
+}
+
+template <typename F, int N = 2>
+inline int32_t dispatch_glynn(int n, const tb::cpx<F> *A, int64_t mstride, int batch, tb::cd *out,
+                       tb::cd *partial, hipStream_t s) {
+  if constexpr (N > TB_GLYNN_MAX_N) {
+    return tbh::fail(TB_ERANGE, "glynn: n=%d exceeds TB_GLYNN_MAX_N=%d", n, TB_GLYNN_MAX_N);
+  } else {
+    if (n == N) return launch_glynn_n<N, F>(A, mstride, batch, out, partial, s);
+    return dispatch_glynn<F, N + 1>(n, A, mstride, batch, out, partial, s);
+  }
+}
+
+int32_t glynn_dispatch_f64(int n, const tb::cpx<double> *A, int64_t mstride, int batch,
+                           tb::cd *out, tb::cd *partial, hipStream_t s);
+int32_t glynn_dispatch_f32(int n, const tb::cpx<float> *A, int64_t mstride, int batch,
+                           tb::cd *out, tb::cd *partial, hipStream_t s);
+
+}  // namespace tbgl

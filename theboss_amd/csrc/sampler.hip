@@ -1,0 +1,439 @@
+// Batched generalized Clifford & Clifford (GCC-B) sampler on the device.
+//
+// Restates GeneralizedCliffordsBSimulationStrategy
+// (theboss/simulation_strategies/generalized_cliffords_b_simulation_strategy.py:41-110) and
+// its uniform-loss variant
+// (generalized_cliffords_b_uniform_losses_simulation_strategy.py:50-121) for S samples at
+// once. Per sample, step k = 1..l:
+//   1. the next input particle joins the current input (the reference pops a uniformly
+//      random remaining particle, :102-105; here a per-sample uniformly random order);
+//   2. the k minors perm(U[r, s - e_nu]) of the current input s (k particles) and output
+//      r (k-1 particles) come from one C&C sweep (:73-77);
+//   3. Laplace expansion amp_m = sum_nu U[m][in_nu] * minor_nu over all m output modes,
+//      pmf = |amp|^2 / sum (:80-92);
+//   4. the output mode is drawn by inverse CDF (numpy.random.choice, :107-110).
+// Samples are processed in lock-step per k. Uniform loss draws l ~ Binomial(n, eta) per
+// sample by CDF threshold (:67-85); samples are ordered by l (largest first), so the
+// active samples at step k are a prefix whose length is read from device memory by the
+// persistent sweep and step kernels. The whole sequence is captured once into a
+// hipGraph and replayed; per-call values (seed, eta) live in device memory.
+//
+// Randomness: Philox-4x32-10 (Salmon et al., SC'11) keyed by the seed, counter
+// (sample, step, purpose). Callers that need the reference's exact draws pass
+// forced orders / uniforms / step counts instead.
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "tb_common.hpp"
+
+namespace tbs {
+size_t sweep_partial_bytes(int k, int items, int c_override);
+int32_t run_sweep(int k, const tb::cd *Vt, int ldv, int64_t item_stride, const int *item_map,
+                  const int *item_count, int items, int c_override, tb::cd *partial,
+                  tb::cd *minors, hipStream_t s);
+}  // namespace tbs
+
+namespace tb {
+
+// ---------------------------------------------------------------- Philox-4x32-10
+struct u4 {
+  uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u4 philox(u4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+    const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    c = {hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// Uniform double in [0,1) with 53 random bits.
+__device__ __forceinline__ double philox_uniform(uint64_t seed, uint32_t sample, uint32_t step,
+                                                 uint32_t purpose) {
+  const u4 r = philox({sample, step, purpose, 0x7462u}, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const uint64_t bits = ((uint64_t)(r.x >> 5) << 26) | (uint64_t)(r.y >> 6);
+  return (double)bits * 0x1.0p-53;
+}
+
+enum : uint32_t { kPurposeDraw = 1, kPurposeOrder = 2, kPurposeLoss = 3 };
+
+// Per-call parameters kept in device memory so a captured graph replays unchanged.
+struct GccbParams {
+  uint64_t seed;
+  double eta;
+};
+
+__global__ void gccb_set_params_kernel(GccbParams *p, uint64_t seed, double eta) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    p->seed = seed;
+    p->eta = eta;
+  }
+}
+
+__device__ __forceinline__ double binom_d(int n, int k) {
+  if (k < 0 || k > n) return 0.0;
+  if (k > n - k) k = n - k;
+  double r = 1.0;
+  for (int i = 1; i <= k; ++i) r = r * (double)(n - k + i) / (double)i;
+  return rint(r);
+}
+
+// Per sample: particle order, number of steps, zeroed output row.
+__global__ void gccb_init_kernel(const GccbParams *__restrict__ prm, int m, int n, int S,
+                                 const int *__restrict__ forced_order,
+                                 const int *__restrict__ forced_steps, int *__restrict__ order,
+                                 int *__restrict__ steps, int *__restrict__ out_samples) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  const uint64_t seed = prm->seed;
+  const double eta = prm->eta;
+  int *ord = order + (size_t)s * n;
+  if (forced_order) {
+    for (int i = 0; i < n; ++i) ord[i] = forced_order[(size_t)s * n + i];
+  } else {
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    for (int i = n - 1; i >= 1; --i) {  // Fisher-Yates
+      const double u = philox_uniform(seed, (uint32_t)s, (uint32_t)i, kPurposeOrder);
+      int j = (int)(u * (double)(i + 1));
+      j = j > i ? i : j;
+      const int t = ord[i];
+      ord[i] = ord[j];
+      ord[j] = t;
+    }
+  }
+  int l;
+  if (forced_steps) {
+    l = forced_steps[s];
+  } else if (eta >= 1.0) {
+    l = n;
+  } else {
+    // generalized_cliffords_b_uniform_losses_simulation_strategy.py:56-85
+    const double u = philox_uniform(seed, (uint32_t)s, 0u, kPurposeLoss);
+    double acc = 0.0;
+    l = n;
+    for (int k = 0; k <= n; ++k) {
+      acc += binom_d(n, k) * pow(eta, (double)k) * pow(1.0 - eta, (double)(n - k));
+      if (acc > u) {
+        l = k;
+        break;
+      }
+    }
+  }
+  l = l < 0 ? 0 : (l > n ? n : l);
+  steps[s] = l;
+  for (int mm = 0; mm < m; ++mm) out_samples[(size_t)s * m + mm] = 0;
+}
+
+// Single workgroup: slots ordered by steps (descending, stable) and count[k] = number of
+// samples with steps >= k, k = 0..n.
+__global__ __launch_bounds__(1024) void gccb_order_kernel(const int *__restrict__ steps, int S,
+                                                          int n, int *__restrict__ slot_map,
+                                                          int *__restrict__ count) {
+  __shared__ int scan[1024];
+  __shared__ int base;
+  const int tid = threadIdx.x;
+  if (tid == 0) base = 0;
+  __syncthreads();
+  for (int v = n; v >= 0; --v) {
+    for (int start = 0; start < S; start += 1024) {
+      const int s = start + tid;
+      const int flag = (s < S && steps[s] == v) ? 1 : 0;
+      scan[tid] = flag;
+      __syncthreads();
+      for (int off = 1; off < 1024; off <<= 1) {
+        const int add = tid >= off ? scan[tid - off] : 0;
+        __syncthreads();
+        scan[tid] += add;
+        __syncthreads();
+      }
+      if (flag) slot_map[base + scan[tid] - 1] = s;
+      __syncthreads();
+      if (tid == 0) base += scan[1023];
+      __syncthreads();
+    }
+    if (tid == 0) count[v] = base;  // samples with steps >= v
+    __syncthreads();
+  }
+}
+
+// U^T restricted to input particles: UinT[nu][mode] = U[mode][particles[nu]].
+__global__ void gccb_uin_kernel(const cd *__restrict__ U, int ldu, const int *__restrict__ particles,
+                                int m, int n, cd *__restrict__ UinT) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= m * n) return;
+  const int nu = idx / m, mm = idx - nu * m;
+  UinT[idx] = U[(size_t)mm * ldu + particles[nu]];
+}
+
+// One wave per active slot at step k: Laplace over m modes, inverse-CDF draw, output and
+// column bookkeeping for the next step.
+__global__ __launch_bounds__(64) void gccb_step_kernel(
+    const GccbParams *__restrict__ prm, int k, int m, int n, int S, const int *__restrict__ count,
+    const int *__restrict__ slot_map, const cd *__restrict__ minors, const cd *__restrict__ UinT,
+    const int *__restrict__ order, const double *__restrict__ forced_uniforms,
+    int *__restrict__ outs, cd *__restrict__ Vt, int *__restrict__ out_samples,
+    cd *__restrict__ trace) {
+  extern __shared__ char smem[];
+  cd *mi = (cd *)smem;                 // k minors
+  double *pm = (double *)(mi + n);     // m probabilities
+  const int slot = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (slot >= count[k]) return;
+  const int s = slot_map[slot];
+  const int *ord = order + (size_t)s * n;
+  if (lane < k) mi[lane] = (k == 1) ? cd{1.0, 0.0} : minors[(size_t)slot * k + lane];
+  __syncthreads();
+  if (trace && lane < k) trace[((size_t)s * n + (k - 1)) * n + lane] = mi[lane];
+  // pmf: amp_m = sum_nu U[m][in_nu] minor_nu over the current k particles.
+  const int per = (m + 63) / 64;
+  const int m0 = lane * per;
+  const int m1 = min(m, m0 + per);
+  double local = 0.0;
+  for (int mm = m0; mm < m1; ++mm) {
+    double re = 0.0, im = 0.0;
+    for (int nu = 0; nu < k; ++nu) {
+      const cd u = UinT[(size_t)ord[nu] * m + mm];
+      const cd v = mi[nu];
+      re += u.re * v.re - u.im * v.im;
+      im += u.re * v.im + u.im * v.re;
+    }
+    const double p = re * re + im * im;
+    pm[mm] = p;
+    local += p;
+  }
+  // Inclusive scan of lane sums.
+  double incl = local;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  const double total = __shfl(incl, 63, 64);
+  const double u = forced_uniforms ? forced_uniforms[(size_t)s * n + (k - 1)]
+                                   : philox_uniform(prm->seed, (uint32_t)s, (uint32_t)k, kPurposeDraw);
+  const double target = u * total;
+  const double excl = incl - local;
+  // The lane whose range holds the target picks the first mode with running sum > target.
+  int pick = -1;
+  if (excl <= target && target < incl) {
+    double run = excl;
+    for (int mm = m0; mm < m1; ++mm) {
+      run += pm[mm];
+      if (run > target) {
+        pick = mm;
+        break;
+      }
+    }
+  }
+  // Resolve (first lane wins); rounding at the very top falls back to the last mode with
+  // nonzero probability.
+  const unsigned long long has = __ballot(pick >= 0);
+  int chosen;
+  if (has) {
+    const int src = __ffsll((long long)has) - 1;
+    chosen = __shfl(pick, src, 64);
+  } else {
+    int last = -1;
+    for (int mm = m0; mm < m1; ++mm)
+      if (pm[mm] > 0.0) last = mm;
+    int best = last;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) best = max(best, __shfl_xor(best, off, 64));
+    chosen = best < 0 ? 0 : best;
+  }
+  if (lane == 0) {
+    outs[(size_t)s * n + (k - 1)] = chosen;
+    out_samples[(size_t)s * m + chosen] += 1;
+  }
+  // New output row j = k-1 for every input particle: Vt[s][nu][k-1] = U[chosen][in_nu].
+  for (int nu = lane; nu < n; nu += 64)
+    Vt[((size_t)s * n + nu) * n + (k - 1)] = UinT[(size_t)ord[nu] * m + chosen];
+}
+
+// Vt rows follow the sample's particle order: Vt[s][nu][j] uses particle ord[nu]. The
+// step kernel writes row entries through ord, and the sweep reads the first k rows.
+
+}  // namespace tb
+
+// ------------------------------------------------------------------------------------
+// Host side: workspace layout, launch sequence, graph cache.
+// ------------------------------------------------------------------------------------
+namespace {
+
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct Layout {
+  size_t params, uint, order, steps, slot_map, count, outs, vt, minors, partial, total;
+};
+
+Layout layout(const tb_gccb_config &c) {
+  const size_t S = (size_t)c.samples, n = (size_t)c.n, m = (size_t)c.m;
+  Layout L{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += align256(bytes);
+    return at;
+  };
+  L.params = take(sizeof(tb::GccbParams));
+  L.uint = take(n * m * sizeof(tb::cd));
+  L.order = take(S * n * sizeof(int));
+  L.steps = take(S * sizeof(int));
+  L.slot_map = take(S * sizeof(int));
+  L.count = take((n + 2) * sizeof(int));
+  L.outs = take(S * n * sizeof(int));
+  L.vt = take(S * n * n * sizeof(tb::cd));
+  L.minors = take(S * n * sizeof(tb::cd));
+  size_t part = 0;
+  for (int k = 2; k <= c.n; ++k) part = std::max(part, tbs::sweep_partial_bytes(k, c.samples, c.chunk_bits[k]));
+  L.partial = take(part);
+  L.total = off;
+  return L;
+}
+
+int32_t enqueue(const tb_gccb_config &c, const tb::cd *U, int ldu, const int *particles,
+                const int *forced_order, const double *forced_uniforms, const int *forced_steps,
+                int *out_samples, tb::cd *trace, char *ws, hipStream_t s) {
+  const Layout L = layout(c);
+  const int S = c.samples, n = c.n, m = c.m;
+  tb::GccbParams *prm = (tb::GccbParams *)(ws + L.params);
+  tb::cd *UinT = (tb::cd *)(ws + L.uint);
+  int *order = (int *)(ws + L.order);
+  int *steps = (int *)(ws + L.steps);
+  int *slot_map = (int *)(ws + L.slot_map);
+  int *count = (int *)(ws + L.count);
+  int *outs = (int *)(ws + L.outs);
+  tb::cd *Vt = (tb::cd *)(ws + L.vt);
+  tb::cd *minors = (tb::cd *)(ws + L.minors);
+  tb::cd *partial = (tb::cd *)(ws + L.partial);
+  hipLaunchKernelGGL(tb::gccb_uin_kernel, dim3((m * n + 255) / 256), dim3(256), 0, s, U, ldu,
+                     particles, m, n, UinT);
+  hipLaunchKernelGGL(tb::gccb_init_kernel, dim3((S + 255) / 256), dim3(256), 0, s, prm, m, n, S,
+                     forced_order, forced_steps, order, steps, out_samples);
+  hipLaunchKernelGGL(tb::gccb_order_kernel, dim3(1), dim3(1024), 0, s, steps, S, n, slot_map,
+                     count);
+  int32_t rc = tbh::check_hip(hipGetLastError(), "gccb_init");
+  if (rc) return rc;
+  const size_t step_lds = (size_t)n * sizeof(tb::cd) + (size_t)m * sizeof(double);
+  for (int k = 1; k <= n; ++k) {
+    if (k >= 2) {
+      rc = tbs::run_sweep(k, Vt, n, (int64_t)n * n, slot_map, count + k, S, c.chunk_bits[k],
+                          partial, minors, s);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(tb::gccb_step_kernel, dim3(S), dim3(64), step_lds, s, prm, k, m, n, S, count,
+                       slot_map, minors, UinT, order, forced_uniforms, outs, Vt, out_samples,
+                       trace);
+    rc = tbh::check_hip(hipGetLastError(), "gccb_step_kernel");
+    if (rc) return rc;
+  }
+  return TB_OK;
+}
+
+// Graph cache: one instantiated graph per distinct launch configuration.
+struct Key {
+  int device, m, n, S, ldu;
+  std::vector<int> chunk;
+  const void *U, *particles, *fo, *fu, *fs, *out, *trace, *ws;
+  bool operator<(const Key &o) const {
+    return std::tie(device, m, n, S, ldu, chunk, U, particles, fo, fu, fs, out, trace, ws) <
+           std::tie(o.device, o.m, o.n, o.S, o.ldu, o.chunk, o.U, o.particles, o.fo, o.fu, o.fs,
+                    o.out, o.trace, o.ws);
+  }
+};
+
+std::mutex g_mu;
+std::map<Key, hipGraphExec_t> g_graphs;
+std::map<int, hipStream_t> g_capture_streams;
+
+}  // namespace
+
+extern "C" size_t tb_gccb_workspace_size(const tb_gccb_config *cfg) {
+  if (!cfg || cfg->m < 1 || cfg->n < 1 || cfg->samples < 1) return 0;
+  return layout(*cfg).total;
+}
+
+extern "C" int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128 *U, int32_t ldu,
+                                  const int32_t *particles, uint64_t seed, double transmissivity,
+                                  const int32_t *forced_order, const double *forced_uniforms,
+                                  const int32_t *forced_steps, int32_t *out_samples,
+                                  tb_complex128 *trace_minors, void *workspace,
+                                  size_t workspace_bytes, void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!cfg) return tbh::fail(TB_EINVAL, "gccb: cfg is NULL");
+  const tb_gccb_config c = *cfg;
+  if (c.m < 1 || c.n < 1 || c.samples < 1)
+    return tbh::fail(TB_EINVAL, "gccb: m=%d n=%d samples=%d must be >= 1", c.m, c.n, c.samples);
+  if (c.n > TB_SWEEP_MAX_K)
+    return tbh::fail(TB_ERANGE, "gccb: n=%d exceeds TB_SWEEP_MAX_K=%d", c.n, TB_SWEEP_MAX_K);
+  if (c.samples > (1 << 24)) return tbh::fail(TB_ERANGE, "gccb: samples=%d > 2^24", c.samples);
+  for (int k = 0; k <= c.n; ++k)
+    if (c.chunk_bits[k] < 0 || c.chunk_bits[k] > 16)
+      return tbh::fail(TB_EINVAL, "gccb: chunk_bits[%d]=%d outside 0..16", k, c.chunk_bits[k]);
+  if (!U || !particles || !out_samples) return tbh::fail(TB_EINVAL, "gccb: NULL pointer");
+  if (ldu < c.m) return tbh::fail(TB_EINVAL, "gccb: ldu=%d < m=%d", ldu, c.m);
+  if (!(transmissivity >= 0.0 && transmissivity <= 1.0))
+    return tbh::fail(TB_EINVAL, "gccb: transmissivity %g outside [0,1]", transmissivity);
+  const size_t need = tb_gccb_workspace_size(&c);
+  if (!workspace || workspace_bytes < need)
+    return tbh::fail(TB_EWORKSPACE, "gccb: workspace %zu bytes < required %zu", workspace_bytes,
+                     need);
+  const size_t step_lds = (size_t)c.n * sizeof(tb::cd) + (size_t)c.m * sizeof(double);
+  if (step_lds > 64 * 1024) return tbh::fail(TB_ERANGE, "gccb: m=%d too large for LDS pmf", c.m);
+  char *ws = (char *)workspace;
+  const Layout L = layout(c);
+  hipLaunchKernelGGL(tb::gccb_set_params_kernel, dim3(1), dim3(1), 0, s,
+                     (tb::GccbParams *)(ws + L.params), seed, transmissivity);
+  int32_t rc = tbh::check_hip(hipGetLastError(), "gccb_set_params_kernel");
+  if (rc) return rc;
+
+  int dev = 0;
+  rc = tbh::check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  Key key{dev, c.m, c.n, c.samples, ldu,
+          std::vector<int>(c.chunk_bits, c.chunk_bits + c.n + 1),
+          U, particles, forced_order, forced_uniforms, forced_steps, out_samples, trace_minors,
+          workspace};
+  std::lock_guard<std::mutex> lock(g_mu);
+  auto it = g_graphs.find(key);
+  if (it == g_graphs.end()) {
+    hipStream_t cap;
+    auto cs = g_capture_streams.find(dev);
+    if (cs == g_capture_streams.end()) {
+      rc = tbh::check_hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
+      if (rc) return rc;
+      g_capture_streams[dev] = cap;
+    } else {
+      cap = cs->second;
+    }
+    rc = tbh::check_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal),
+                        "hipStreamBeginCapture");
+    if (rc) return rc;
+    int32_t erc = enqueue(c, (const tb::cd *)U, ldu, particles, forced_order, forced_uniforms,
+                          forced_steps, out_samples, (tb::cd *)trace_minors, ws, cap);
+    hipGraph_t graph = nullptr;
+    rc = tbh::check_hip(hipStreamEndCapture(cap, &graph), "hipStreamEndCapture");
+    if (erc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return erc;
+    }
+    if (rc) return rc;
+    hipGraphExec_t exec = nullptr;
+    rc = tbh::check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0),
+                        "hipGraphInstantiate");
+    (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    it = g_graphs.emplace(key, exec).first;
+  }
+  return tbh::check_hip(hipGraphLaunch(it->second, s), "hipGraphLaunch");
+}

@@ -1,0 +1,217 @@
+"""GPU permanent calculators: drop-in counterparts of theboss's calculators whose
+arithmetic runs in the gfx950 kernels of libtheboss_amd.so.
+
+Base-class semantics follow theboss's BSPermanentCalculatorBase
+(theboss/boson_sampling_utilities/permanent_calculators/bs_permanent_calculator_base.py:22-72):
+the matrix and states are stored by reference (None -> empty list), so in-place edits
+such as `calc.matrix *= sqrt(eta)` are seen by the next computation; the shape check
+raises AttributeError (:179-180). Each compute call gathers only the submatrix it needs
+on the host and ships that to the device; no device state is kept between calls, so the
+objects deepcopy and pickle like the reference's.
+"""
+from typing import List, Optional, Sequence
+
+import numpy as np
+from numpy import complex128
+
+from .. import _device as D
+from .. import _lib as L
+from .. import ops
+from ..states import as_occupation, compact, effective_matrix
+from .bs_permanent_calculator_interface import BSPermanentCalculatorInterface
+
+
+class BSPermanentCalculatorBase(BSPermanentCalculatorInterface):
+    def __init__(self, matrix, input_state: Optional[Sequence[int]] = None,
+                 output_state: Optional[Sequence[int]] = None) -> None:
+        if output_state is None:
+            output_state = list()
+        if input_state is None:
+            input_state = list()
+        self._matrix = matrix
+        self._input_state = input_state
+        self._output_state = output_state
+
+    @property
+    def matrix(self):
+        return self._matrix
+
+    @matrix.setter
+    def matrix(self, matrix) -> None:
+        self._matrix = matrix
+
+    @property
+    def input_state(self):
+        return self._input_state
+
+    @input_state.setter
+    def input_state(self, input_state) -> None:
+        self._input_state = input_state
+
+    @property
+    def output_state(self):
+        return self._output_state
+
+    @output_state.setter
+    def output_state(self, output_state) -> None:
+        self._output_state = output_state
+
+    def _can_calculation_be_performed(self) -> bool:
+        """bs_permanent_calculator_base.py:61-72: square matrix, equal state lengths, no
+        more modes than columns."""
+        return (
+            len(self._matrix) == len(self._matrix[0])
+            and len(self._output_state) == len(self._input_state)
+            and len(self._output_state) <= len(self._matrix[0])
+        )
+
+    def _checked_states(self):
+        if self._matrix is None or not self._can_calculation_be_performed():
+            raise AttributeError("matrix and states have incompatible sizes")
+        return as_occupation(self._input_state), as_occupation(self._output_state)
+
+    def compute_permanent_batch(self, input_states, output_states) -> np.ndarray:
+        """Permanents for many (input, output) pairs on the same matrix in one launch
+        (no reference counterpart; the reference loops over compute_permanent)."""
+        raise NotImplementedError
+
+
+def _glynn_on_effective(matrices: List[np.ndarray], precision: int) -> np.ndarray:
+    """Glynn permanents of equal-size square effective matrices (one launch)."""
+    T = D.require_gpu()
+    n = matrices[0].shape[0]
+    A = D.to_device_c128(np.stack(matrices).reshape(len(matrices), n, n))
+    out = ops.glynn_batch(A, precision)
+    return out.cpu().numpy()
+
+
+class GPUGlynnGrayPermanentCalculator(BSPermanentCalculatorBase):
+    """Glynn's formula in Gray order on the GPU
+    (glynn_gray_permanent_calculator.py:41-84). Empty effective matrix -> 1 (:52-53)."""
+
+    precision = L.TB_PREC_FP64
+
+    def compute_permanent(self) -> complex128:
+        s, t = self._checked_states()
+        if s.sum() == 0 or t.sum() == 0:
+            return complex128(1)
+        if s.sum() != t.sum():
+            raise AttributeError("effective matrix is not square (sum(input) != sum(output))")
+        M = effective_matrix(self._matrix, s, t)
+        return complex128(_glynn_on_effective([M], self.precision)[0])
+
+    def compute_permanent_batch(self, input_states, output_states) -> np.ndarray:
+        out = np.ones(len(input_states), dtype=np.complex128)
+        groups = {}
+        for idx, (si, ti) in enumerate(zip(input_states, output_states)):
+            s, t = as_occupation(si), as_occupation(ti)
+            if s.sum() == 0 or t.sum() == 0:
+                continue
+            if s.sum() != t.sum():
+                raise AttributeError("effective matrix is not square")
+            groups.setdefault(int(s.sum()), []).append((idx, effective_matrix(self._matrix, s, t)))
+        for n, items in groups.items():
+            vals = _glynn_on_effective([M for _, M in items], self.precision)
+            for (idx, _), v in zip(items, vals):
+                out[idx] = v
+        return out
+
+
+class GPUGlynnGrayFP32PermanentCalculator(GPUGlynnGrayPermanentCalculator):
+    """Glynn with FP32 column sums and products, FP64 accumulation of the alternating
+    sum (no reference counterpart: a precision/throughput variant)."""
+
+    precision = L.TB_PREC_FP32_COLSUM
+
+
+class _GuanGPUCalculator(BSPermanentCalculatorBase):
+    """Multiplicity-aware Guan-code calculators (Ryser, Chin-Huh) on the GPU."""
+
+    formula = L.TB_FORMULA_RYSER
+
+    def _launch(self, pairs) -> np.ndarray:
+        T = D.require_gpu()
+        U = np.asarray(self._matrix, dtype=np.complex128)
+        subs, out_mults, in_mults = [], [], []
+        for s, t in pairs:
+            om, ot = compact(t)
+            im, it = compact(s)
+            sub = U[np.ix_(om, im)]
+            if np.prod(ot.astype(np.float64) + 1) < np.prod(it.astype(np.float64) + 1):
+                sub, ot, it = sub.T, it, ot  # perm(U_ST) = perm(U^T_TS)
+            subs.append(sub)
+            out_mults.append(ot)
+            in_mults.append(it)
+        J = max(1, max(len(x) for x in out_mults))
+        Dd = max(1, max(len(x) for x in in_mults))
+        B = len(pairs)
+        if J > L.TB_GUAN_MAX_ROWS or Dd > L.TB_GUAN_MAX_COLS:
+            return None
+        stack = np.zeros((B * J, Dd), dtype=np.complex128)
+        out_modes = np.zeros((B, J), dtype=np.int32)
+        out_mult = np.zeros((B, J), dtype=np.int32)
+        in_modes = np.zeros((B, Dd), dtype=np.int32)
+        in_mult = np.zeros((B, Dd), dtype=np.int32)
+        for b in range(B):
+            sub = subs[b]
+            stack[b * J:b * J + sub.shape[0], :sub.shape[1]] = sub
+            out_modes[b] = b * J + np.arange(J)
+            out_mult[b, :len(out_mults[b])] = out_mults[b]
+            in_modes[b] = np.arange(Dd)
+            in_mult[b, :len(in_mults[b])] = in_mults[b]
+        dev = D.device()
+        res = ops.guan_perm_batch(D.to_device_c128(stack, dev), D.to_device_i32(out_modes, dev),
+                                  D.to_device_i32(out_mult, dev), D.to_device_i32(in_modes, dev),
+                                  D.to_device_i32(in_mult, dev), self.formula)
+        return res.cpu().numpy()
+
+    def compute_permanent(self) -> complex128:
+        s, t = self._checked_states()
+        if len(s) == 0:
+            raise AttributeError("empty occupation vectors")
+        return complex128(self.compute_permanent_batch([s], [t])[0])
+
+    def compute_permanent_batch(self, input_states, output_states) -> np.ndarray:
+        pairs = [(as_occupation(s), as_occupation(t)) for s, t in zip(input_states, output_states)]
+        if not pairs:
+            return np.zeros(0, dtype=np.complex128)
+        res = self._launch(pairs)
+        if res is None:
+            # Too many distinct modes for the mixed-radix kernel: expand multiplicities
+            # and use the Glynn kernel on the effective matrices (same permanent).
+            glynn = GPUGlynnGrayPermanentCalculator(self._matrix)
+            res = glynn.compute_permanent_batch([p[0] for p in pairs], [p[1] for p in pairs])
+            for i, (s, t) in enumerate(pairs):
+                if s.sum() == 0 and t.sum() != 0:
+                    res[i] = 0.0
+        return res
+
+
+class GPURyserPermanentCalculator(_GuanGPUCalculator):
+    """Ryser's formula over Guan codes with mode multiplicities
+    (ryser_permanent_calculator.py:45-64 on bs_permanent_calculator_base.py:166-209)."""
+
+    formula = L.TB_FORMULA_RYSER
+
+
+class GPUChinHuhPermanentCalculator(_GuanGPUCalculator):
+    """Chin-Huh (Glynn with multiplicities) over Guan codes
+    (chin_huh_permanent_calculator.py:38-59)."""
+
+    formula = L.TB_FORMULA_CHIN_HUH
+
+
+class GPUClassicPermanentCalculator(BSPermanentCalculatorBase):
+    """Counterpart of ClassicPermanentCalculator (classic_permanent_calculator.py:27-41):
+    same edge values (no input particles -> 1 if no output particles else 0); the
+    permanent itself comes from the GPU Ryser kernel rather than row expansion."""
+
+    def compute_permanent(self) -> complex128:
+        s, t = self._checked_states()
+        if s.sum() == 0:
+            return complex128(1 if t.sum() == 0 else 0)
+        return GPURyserPermanentCalculator(self._matrix, s, t).compute_permanent()
+
+    def compute_permanent_batch(self, input_states, output_states) -> np.ndarray:
+        return GPURyserPermanentCalculator(self._matrix).compute_permanent_batch(
+            input_states, output_states)

@@ -1,0 +1,79 @@
+"""Host-side occupation-vector helpers (no permanent arithmetic here).
+
+Restates the parts of theboss/boson_sampling_utilities/boson_sampling_utilities.py the
+permanent path needs: mode assignment <-> mode occupation (:34-78), the effective
+scattering matrix gather (:545-626) and the 2m loss expansion (:287-342).
+"""
+from typing import Sequence, Tuple
+
+import numpy as np
+
+
+def as_occupation(state) -> np.ndarray:
+    """Occupation vector as int64. Accepts lists, tuples, int or float arrays (the
+    reference's tests pass float arrays, tests/test_bs_submatrices_permanent_calculators.py:48-50);
+    rejects non-integer or negative entries."""
+    a = np.asarray(state)
+    if a.size == 0:
+        return np.zeros(0, dtype=np.int64)
+    f = np.asarray(a, dtype=np.float64).reshape(-1)
+    r = np.round(f)
+    if not np.all(r == f):
+        raise ValueError(f"occupation numbers must be integers, got {state!r}")
+    if np.any(r < 0):
+        raise ValueError(f"occupation numbers must be non-negative, got {state!r}")
+    return r.astype(np.int64)
+
+
+def compact(state: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """(modes, multiplicities) of the occupied modes, in mode order."""
+    modes = np.nonzero(state)[0]
+    return modes.astype(np.int32), state[modes].astype(np.int32)
+
+
+def effective_matrix(matrix, input_state, output_state) -> np.ndarray:
+    """M[a][b] = U[out_mode(a)][in_mode(b)], rows repeated t_j times and columns s_i
+    times (boson_sampling_utilities.py:595-626). Empty (0 x 0) when either side has no
+    particles (:606-607)."""
+    U = np.asarray(matrix, dtype=np.complex128)
+    s, t = as_occupation(input_state), as_occupation(output_state)
+    if s.sum() == 0 or t.sum() == 0:
+        return np.zeros((0, 0), dtype=np.complex128)
+    rows = np.repeat(np.arange(len(t)), t)
+    cols = np.repeat(np.arange(len(s)), s)
+    return np.ascontiguousarray(U[np.ix_(rows, cols)])
+
+
+def mode_occupation_to_mode_assignment(mode_occupation: Sequence[int]) -> Tuple[int, ...]:
+    """boson_sampling_utilities.py:60-78: occupation -> sorted particle modes."""
+    occ = as_occupation(mode_occupation)
+    return tuple(int(x) for x in np.repeat(np.arange(len(occ)), occ))
+
+
+def mode_assignment_to_mode_occupation(modes_assignment: Sequence[int],
+                                       observed_modes_number: int = 0) -> Tuple[int, ...]:
+    """boson_sampling_utilities.py:34-57: particle modes -> occupation vector."""
+    if observed_modes_number == 0:
+        observed_modes_number = max(modes_assignment) + 1
+    occ = np.zeros(observed_modes_number, dtype=np.int64)
+    for mode in modes_assignment:
+        occ[mode] += 1
+    return tuple(int(x) for x in occ)
+
+
+def prepare_interferometer_matrix_in_expanded_space(interferometer_matrix) -> np.ndarray:
+    """Embed a (possibly lossy) m x m interferometer into a 2m x 2m unitary whose extra
+    modes absorb lost particles (boson_sampling_utilities.py:287-342): with the SVD
+    U = V diag(sigma) W, the expanded matrix is
+        [V 0; 0 I] [diag(sigma) diag(sqrt(1-sigma^2)); diag(sqrt(1-sigma^2)) diag(sigma)] [W 0; 0 I].
+    """
+    U = np.asarray(interferometer_matrix, dtype=np.complex128)
+    v, sigma, w = np.linalg.svd(U)
+    m = U.shape[0]
+    z = np.zeros_like(v)
+    eye = np.eye(m)
+    loss = np.sqrt(np.clip(1.0 - sigma ** 2, 0.0, None))
+    ev = np.block([[v, z], [z, eye]])
+    ew = np.block([[w, z], [z, eye]])
+    es = np.block([[np.diag(sigma), np.diag(loss)], [np.diag(loss), np.diag(sigma)]])
+    return ev @ es @ ew
