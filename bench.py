@@ -1,0 +1,188 @@
+"""Benchmark: batched GCC-B sampling at n=20 photons, m=400 modes on one MI355X.
+
+One step = one batched GCC-B sampling pass producing 4096 samples (lossless, input
+photons in modes 0..19, Haar-random 400 x 400 unitary, synthetic and seeded), run as
+one replayed hipGraph. `value` is samples/s of the batched path with the interferometer
+resident in HBM. The JSON line also carries the plain path (one C&C sweep launch per
+sample step, Laplace expansion and draw on the host, the reference's per-step
+structure), the roofline of the dominant kernel (the k=20 C&C sweep, FP64 VALU) and the
+CPU baseline (the oracle's C restatement of the reference sampler on the host cores).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+M, N, SAMPLES, SEED = 400, 20, 4096, 20240611
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector peak
+METRIC = "GCC samples/s n=20 m=400 batched vs plain; perm+grad % of FP64 VALU peak"
+
+
+def sweep_flops(k: int) -> float:
+    """Algorithmic flops of one C&C sweep over k collision-free particles and k-1 output
+    rows: (10k - 12) per term (k-1 complex adds for the column update, k-2 complex
+    multiplies for the product, the running sum and the minors' share; complex add = 2,
+    complex mul = 6) times the 2^(k-1) terms of the Glynn-type minors sum."""
+    return float(2 ** (k - 1) * (10 * k - 12))
+
+
+def sample_flops(n: int, m: int) -> float:
+    """One GCC-B sample: sum_k [sweep(k) + 8 m k] (Laplace over m modes)."""
+    return float(sum(sweep_flops(k) + 8 * m * k for k in range(1, n + 1)))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-samples", type=int, default=24)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="only run the k=20 sweep twice and a 256 MiB device copy (the "
+                         "calibration reference) for rocprofv3 --pmc passes")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(sampling would shard samples across GPUs with no exchange).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from oracle import oracle as orc  # checker + CPU baseline only
+    from theboss_amd import _device as D
+    from theboss_amd import ops
+    from theboss_amd.permanent_calculators import GPURyserPermanentCalculator
+    from theboss_amd.simulation_strategies import GeneralizedCliffordsBSimulationStrategy
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    U = orc.haar_unitary(M, SEED)
+    Ud = D.to_device_c128(U, dev)
+    if args.pmc_probe:
+        Vt = Ud[:N, :N].T.contiguous().unsqueeze(0).expand(SAMPLES, N, N).contiguous()
+        for _ in range(2):
+            ops.sweep_minors_batch(Vt, N)
+        a = torch.ones(256 * 1024 * 1024 // 4, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(2):
+            b.copy_(a)  # 256 MiB read + 256 MiB write: scales FETCH_SIZE / WRITE_SIZE
+        torch.cuda.synchronize()
+        print("pmc probe done")
+        return 0
+    particles = D.to_device_i32(np.arange(N), dev)
+    out = torch.empty((SAMPLES, M), dtype=torch.int32, device=dev)
+
+    # ---- batched path: warmup then exactly K timed steps
+    seeds = [SEED + i for i in range(args.warmup + args.steps)]
+    for i in range(args.warmup):
+        ops.gccb_sample(Ud, particles, SAMPLES, seeds[i], out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ops.gccb_sample(Ud, particles, SAMPLES, seeds[args.warmup + i], out=out)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    assert int(out.sum().item()) == SAMPLES * N, "sampler lost photons"
+    batched = SAMPLES * args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+
+    # ---- plain path: one sweep launch per (sample, step), host Laplace + draw
+    plain = GeneralizedCliffordsBSimulationStrategy(GPURyserPermanentCalculator(U),
+                                                    rng=np.random.default_rng(SEED))
+    inp = [1] * N + [0] * (M - N)
+    plain.simulate(inp, 2)  # warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain.simulate(inp, args.plain_samples)
+    torch.cuda.synchronize()
+    plain_rate = args.plain_samples / (time.perf_counter() - t0)
+
+    # ---- roofline of the dominant kernel: the k=20 sweep over the 4096 samples' columns
+    k = N
+    Vt = torch.empty((SAMPLES, N, N), dtype=torch.complex128, device=dev)
+    Vt.copy_(Ud[:N, :N].T.contiguous().unsqueeze(0).expand(SAMPLES, N, N))
+    ops.sweep_minors_batch(Vt, k)
+    torch.cuda.synchronize()
+    reps = 5
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        ops.sweep_minors_batch(Vt, k)
+    e1.record()
+    torch.cuda.synchronize()
+    t_launch = e0.elapsed_time(e1) / 1e3 / reps
+    achieved = SAMPLES * sweep_flops(k) / t_launch / 1e12
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "sweep_k20_hbm_traffic.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    # ---- CPU baseline: the oracle's GCC-B sampler on the host cores, bounded sample
+    cpu = None
+    if not args.no_cpu_baseline:
+        cores = max(1, min(16, os.cpu_count() or 1))
+        rng = np.random.default_rng(SEED)
+        done, spent, batch = 0, 0.0, cores
+        while spent < args.cpu_seconds and done < 512:
+            order = np.stack([rng.permutation(N) for _ in range(batch)]).astype(np.int32)
+            unif = rng.random((batch, N))
+            t0 = time.perf_counter()
+            orc.gccb_sample_batch(U, order, unif, threads=cores)
+            spent += time.perf_counter() - t0
+            done += batch
+        cpu = {"value": done / spent, "unit": "samples/s", "cores": cores, "kind": "port",
+               "sample": f"{done} GCC-B samples at n={N}, m={M} (same Haar matrix) by the "
+                         f"oracle's C restatement of the reference sampler, OpenMP over samples"}
+
+    line = {
+        "metric": METRIC,
+        "value": batched,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "GCC-B batched sampling (lossless), m=400 modes, n=20 photons in modes "
+                        "0..19, Haar-random complex128 unitary (seeded QR), 4096 samples per step",
+            "m": M, "n": N, "samples_per_step": SAMPLES, "seed": SEED,
+            "parallelism": "single GPU",
+        },
+        "plain_samples_per_s": plain_rate,
+        "batched_vs_plain": batched / plain_rate,
+        "fp64_flops_per_sample": sample_flops(N, M),
+        "fp64_tflops_sampler": batched * sample_flops(N, M) / 1e12,
+        "roofline": {
+            "kernel": "sweep_kernel<20,3,1> (C&C minors sweep, k=20, 4096 items)",
+            "flops_per_launch": SAMPLES * sweep_flops(k),
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": traffic,
+            "launch_ms": t_launch * 1e3,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -40,7 +40,10 @@ extern "C" {
 #ifndef TB_GLYNN_MAX_N
 #define TB_GLYNN_MAX_N 48   /* largest square matrix for tb_glynn_batch */
 #endif
-#define TB_SWEEP_MAX_K 40   /* largest particle count for the binary C&C sweep */
+#ifndef TB_SWEEP_MAX_K
+#define TB_SWEEP_MAX_K 40
+#endif
+#define TB_SWEEP_MAX_K_DOC   /* largest particle count for the binary C&C sweep */
 #define TB_GUAN_MAX_ROWS 48 /* distinct output modes for the mixed-radix kernels */
 #define TB_GUAN_MAX_COLS 32 /* distinct input modes for the mixed-radix kernels */
 

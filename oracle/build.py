@@ -18,6 +18,9 @@ def build(verbose: bool = False) -> str:
     cmd = [
         "gcc", "-O2", "-std=c11", "-fopenmp", "-fPIC", "-shared",
         "-fno-fast-math", "-ffp-contract=off",
+        # numpy multiplies complex128 with the plain (a*c - b*d, a*d + b*c) formula; so does
+        # the oracle (no C99 Annex G inf/nan recovery calls).
+        "-fcx-limited-range",
         SRC, "-o", LIB + ".tmp", "-lm",
     ]
     if verbose:

@@ -23,11 +23,17 @@ How each routine is pinned (tests/test_oracle.py):
 * ryser / chin-huh     -- equal classic on the nine occupation patterns of the
                           reference's tests/test_bs_permanent_calculators.py:88-122
                           and on random multiplicity patterns.
-* cc submatrices       -- every entry equals the classic permanent of U[t, s-e_i]
+* cc submatrices       -- (Ryser restatement, the reference's algorithm) every entry
+                          equals the classic permanent of U[t, s-e_i]
                           (the reference compares absolute values only,
                           tests/test_bs_submatrices_permanent_calculators.py:75-109;
                           the check here compares complex values), plus the Laplace
                           identity sum_i s_i U[m][i] minor_i = perm(U[t+e_m, s]).
+                          The Ryser walk's alternating subset sums cancel strongly:
+                          at k = 20 its own relative error reaches ~1e-10 (measured
+                          against cc_submatrices_glynn), so large-k checks of the GPU
+                          sweep use cc_submatrices_glynn / perm_minors (Glynn with
+                          double-double accumulation, pinned like glynn above).
 * minors               -- Laplace identities sum_j A_ij G_ij = perm(A) for every i and
                           classic minors for n <= 7.
 * gccb pmf             -- equals |classic perm(U[r+e_m, s])|^2 normalised.
@@ -413,3 +419,16 @@ def gcca_layer_pmf(U, input_state, r_sample, perm=perm_ryser) -> np.ndarray:
             acc += p * w
         pmf[o] = acc
     return pmf
+
+
+def cc_submatrices_glynn(U, s, t) -> np.ndarray:
+    """Oracle-grade minors for large k: each perm(U[t, s - e_i]) by Glynn with re-seeded
+    column sums and double-double accumulation (mode 1). Independent of the Ryser walk,
+    whose alternating subset sums lose ~1e-10 relative accuracy by k = 20."""
+    s = _ci(s)
+    out = np.zeros(len(s), dtype=np.complex128)
+    for i in np.nonzero(s)[0]:
+        sm = s.copy()
+        sm[i] -= 1
+        out[i] = perm_glynn(U, sm, t, 1)
+    return out

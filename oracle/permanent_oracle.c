@@ -419,7 +419,7 @@ void orc_perm_minors(const cplx *A, int n, int threads, cplx *G) {
       }
       ++a;
     }
-    G[ij] = orc_perm_glynn_matrix(sub, n - 1, n - 1 >= 20 ? 1 : 0);
+    G[ij] = orc_perm_glynn_matrix(sub, n - 1, 1);
   }
 }
 

@@ -178,8 +178,13 @@ def test_submatrices_binary_sweep_vs_oracle(T, oracle, k):
     s[rng.choice(m, k, replace=False)] = 1
     t = np.bincount(rng.integers(0, m, k - 1), minlength=m)  # outputs may collide
     got = np.asarray(GPUBSCCRyserSubmatricesPermanentCalculator(U, s, t).compute_permanents())
-    ref = oracle.cc_submatrices(U, s, t)
+    # Oracle-grade minors (Glynn, double-double); the Ryser restatement of the reference
+    # (cc_submatrices) carries ~1e-10 error of its own at k = 20, so it is held to 1e-9.
+    ref = oracle.cc_submatrices_glynn(U, s, t) if k >= 12 else oracle.cc_submatrices(U, s, t)
     assert rel(got, ref) <= REL, (k, rel(got, ref))
+    occ = np.nonzero(s)[0]
+    assert np.max(np.abs(got[occ] - ref[occ]) / np.abs(ref[occ])) <= REL  # every minor
+    assert rel(got, oracle.cc_submatrices(U, s, t)) <= (1e-9 if k >= 16 else REL)
 
 
 def test_submatrices_collision_inputs_vs_oracle(T, oracle):
@@ -323,3 +328,37 @@ def test_errors_surface_as_exceptions(T, ops):
     A = T.zeros((1, 49, 49), dtype=T.complex128, device="cuda")
     with pytest.raises(TbError):
         ops.glynn_batch(A)
+
+
+def test_batched_sampler_bench_shape_minors_vs_oracle(T, ops, oracle):
+    """m = 400, n = 20 (the benchmark workload) with forced draws: the samples equal the
+    oracle sampler's, and the traced minors of the last steps equal the oracle-grade
+    minors of the same (input, output) prefix at <= 1e-10 relative error, as do the step
+    amplitudes perm(U[r + e_o, s]) obtained by the Laplace expansion."""
+    m, n, S = 400, 20, 3
+    rng = np.random.default_rng(400)
+    U = oracle.haar_unitary(m, 20240611)
+    order, unif = _forced(rng, S, n)
+    out, trace = ops.gccb_sample(dev_c(T, U), dev_i(T, np.arange(n)), S, 1, 1.0, dev_i(T, order),
+                                 T.from_numpy(unif).cuda(), None, trace=True)
+    out, tr = out.cpu().numpy(), trace.cpu().numpy()
+    expect = oracle.gccb_sample_batch(U, order, unif)
+    assert np.array_equal(out, expect)
+    s_idx = 0
+    cur = np.zeros(m, int)
+    r = np.zeros(m, int)
+    for k in range(1, n + 1):
+        cur[order[s_idx, k - 1]] += 1
+        if k in (n - 1, n):
+            ref = oracle.cc_submatrices_glynn(U, cur, r)
+            got = tr[s_idx, k - 1, :k]
+            want = np.array([ref[order[s_idx, nu]] for nu in range(k)])
+            assert np.max(np.abs(got - want) / np.abs(want)) <= REL, k
+            o = int(np.argmax(np.abs(U[:, :n] @ (cur[:n] * ref[:n]))))
+            amp = np.sum(U[o, order[s_idx, :k]] * got)
+            to = r.copy()
+            to[o] += 1
+            ref_amp = oracle.perm_glynn(U, cur, to, 1)
+            assert abs(amp - ref_amp) <= REL * abs(ref_amp), k
+        pmf, _ = oracle.gccb_pmf(U, cur, r)
+        r[int(np.searchsorted(np.cumsum(pmf), unif[s_idx, k - 1], side="right"))] += 1

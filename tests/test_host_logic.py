@@ -1,0 +1,152 @@
+"""CPU tests of the host-side logic (no GPU, no kernel calls)."""
+import math
+import pickle
+from copy import deepcopy
+
+import numpy as np
+import pytest
+
+
+def test_as_occupation_accepts_reference_input_forms():
+    from theboss_amd.states import as_occupation
+
+    assert list(as_occupation([1, 0, 2])) == [1, 0, 2]
+    assert list(as_occupation(np.array([1.0, 3.0, 0.0]))) == [1, 3, 0]
+    assert list(as_occupation((0, 1))) == [0, 1]
+    with pytest.raises(ValueError):
+        as_occupation([0.5, 1])
+    with pytest.raises(ValueError):
+        as_occupation([-1, 1])
+
+
+def test_compact_batch_order_and_padding():
+    from theboss_amd.states import compact_batch
+
+    S = np.array([[0, 2, 0, 1], [0, 0, 0, 0], [3, 0, 0, 0]])
+    modes, mult = compact_batch(S)
+    assert modes.shape == (3, 2)
+    assert list(modes[0]) == [1, 3] and list(mult[0]) == [2, 1]
+    assert list(mult[1]) == [0, 0]
+    assert modes[2][0] == 0 and list(mult[2]) == [3, 0]
+
+
+def test_effective_matrix_matches_oracle(oracle):
+    from theboss_amd.states import effective_matrix
+
+    U = oracle.haar_unitary(5, 2)
+    s, t = [0, 2, 1, 0, 0], [1, 0, 0, 2, 0]
+    assert np.array_equal(effective_matrix(U, s, t), oracle.effective_matrix(U, s, t))
+    assert effective_matrix(U, [0] * 5, t).shape == (0, 0)
+
+
+def test_mode_assignment_round_trip():
+    from theboss_amd.states import mode_assignment_to_mode_occupation, mode_occupation_to_mode_assignment
+
+    occ = (0, 2, 1, 0, 3)
+    assign = mode_occupation_to_mode_assignment(occ)
+    assert assign == (1, 1, 2, 4, 4, 4)
+    assert mode_assignment_to_mode_occupation(assign, 5) == occ
+
+
+def test_expanded_space_first_columns_embed_lossy_matrix(oracle):
+    from theboss_amd.states import prepare_interferometer_matrix_in_expanded_space
+
+    m = 5
+    U = oracle.haar_unitary(m, 4) @ np.diag(np.sqrt([0.9, 0.5, 0.7, 1.0, 0.3]))
+    E = prepare_interferometer_matrix_in_expanded_space(U)
+    assert E.shape == (2 * m, 2 * m)
+    assert np.allclose(E[:m, :m], U, atol=1e-12)
+    cols = E[:, :m]
+    assert np.allclose(cols.conj().T @ cols, np.eye(m), atol=1e-12)
+
+
+def test_input_substates_match_reference_order(oracle):
+    from theboss_amd.simulation_strategies.generalized_cliffords_simulation_strategy import input_substates
+
+    for s in ([1, 2, 0], [2, 1, 1, 0], [1, 1, 1, 1]):
+        assert input_substates(s) == oracle._input_substates(s)
+
+
+def test_clements_layout_counts():
+    from theboss_amd.interferometer import clements_layout
+
+    assert len(clements_layout(16)) == 120
+    for m in (2, 3, 6, 9):
+        units = clements_layout(m)
+        assert len(units) == m * (m - 1) // 2
+        assert all(0 <= k < m - 1 for _, k in units)
+
+
+def test_clements_unitary_cpu_is_unitary():
+    import torch
+
+    from theboss_amd.interferometer import clements_unitary
+
+    g = torch.Generator().manual_seed(0)
+    th = torch.rand(28, generator=g, dtype=torch.float64)
+    ph = torch.rand(28, generator=g, dtype=torch.float64)
+    U = clements_unitary(th, ph, 8)
+    assert torch.allclose(U.conj().T @ U, torch.eye(8, dtype=torch.complex128), atol=1e-12)
+
+
+def test_calculators_deepcopy_and_pickle_without_gpu(oracle):
+    """The strategy factory deep-copies calculators and spawn pools pickle them
+    (simulation_strategy_factory.py:58; nonuniform_losses_approximation_strategy.py:254-257):
+    the GPU calculators hold only host data."""
+    from theboss_amd.permanent_calculators import BSPermanentCalculatorFactory, PermanentCalculatorType
+
+    U = oracle.haar_unitary(4, 1)
+    for t in PermanentCalculatorType:
+        calc = BSPermanentCalculatorFactory(U, [1, 1, 0, 0], [0, 1, 1, 0], t).generate_calculator()
+        c2 = deepcopy(calc)
+        c3 = pickle.loads(pickle.dumps(calc))
+        assert np.array_equal(c2.matrix, U) and np.array_equal(c3.matrix, U)
+
+
+def test_factory_enum_values_and_default(oracle):
+    from theboss_amd.permanent_calculators import (
+        BSPermanentCalculatorFactory, GPUChinHuhPermanentCalculator, GPURyserPermanentCalculator,
+        PermanentCalculatorType)
+
+    assert [int(t) for t in PermanentCalculatorType][:4] == [1, 2, 3, 4]
+    f = BSPermanentCalculatorFactory(None, None, None)
+    assert isinstance(f.generate_calculator(), GPURyserPermanentCalculator)
+    f2 = BSPermanentCalculatorFactory(None, None, None, calculator_type=99)
+    assert isinstance(f2.generate_calculator(), GPUChinHuhPermanentCalculator)
+
+
+def test_calculator_stores_references():
+    from theboss_amd.permanent_calculators import GPURyserPermanentCalculator
+
+    U = np.eye(3, dtype=complex)
+    calc = GPURyserPermanentCalculator(U)
+    calc.matrix *= 0.5  # the reference's tests mutate the matrix in place
+    assert calc.matrix is U and U[0, 0] == 0.5
+    assert calc.input_state == [] and calc.output_state == []
+
+
+def test_shape_check_raises_attribute_error_before_gpu():
+    from theboss_amd.permanent_calculators import GPUGlynnGrayPermanentCalculator
+
+    calc = GPUGlynnGrayPermanentCalculator(np.eye(3, dtype=complex), [1, 1, 1], [1, 1])
+    with pytest.raises(AttributeError):
+        calc.compute_permanent()
+
+
+def test_bench_flop_model():
+    import bench
+
+    assert bench.sweep_flops(20) == 2 ** 19 * 188
+    # one sample: sum_k [2^(k-1)(10k-12) + 8 m k]
+    expect = sum(2 ** (k - 1) * (10 * k - 12) + 8 * 400 * k for k in range(1, 21))
+    assert bench.sample_flops(20, 400) == expect
+
+
+def test_state_enumeration_matches_oracle(oracle):
+    from theboss_amd.states import generate_lossy_n_particle_input_states, generate_possible_states
+
+    for n, m, losses in [(2, 4, False), (3, 5, True), (0, 3, False), (4, 2, True)]:
+        assert generate_possible_states(n, m, losses) == oracle.generate_possible_states(n, m, losses)
+    for init, left in [([1, 1, 1, 0, 0], 1), ([2, 1, 0], 2), ([1, 2, 1], 3)]:
+        assert generate_lossy_n_particle_input_states(init, left) == \
+            oracle.generate_lossy_n_particle_input_states(init, left)

@@ -8,7 +8,7 @@ import os
 import threading
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtheboss_amd.so")
+LIB_PATH = os.environ.get("THEBOSS_AMD_LIB") or os.path.join(HERE, "libtheboss_amd.so")
 
 TB_OK = 0
 TB_EINVAL = -1

@@ -26,7 +26,7 @@ namespace tb {
 // step (which otherwise multiplies live registers by the unroll factor and spills).
 // Occupancy target for the wide kernels: 2 waves per SIMD (<= 256 VGPRs).
 #ifndef TB_WIDE_ATTR
-#define TB_WIDE_ATTR __attribute__((amdgpu_waves_per_eu(2)))
+#define TB_WIDE_ATTR(N) __attribute__((amdgpu_waves_per_eu((N) <= 40 ? 2 : 1)))
 #endif
 
 #ifndef TB_STEP_FENCE
@@ -198,20 +198,8 @@ __device__ __forceinline__ void glynn_lane(P M, int C, uint64_t q0, uint64_t qst
       cim = (double)p.im;
     }
     const int nblocks = 1 << (C - U);
-    for (int blk = 0; blk < nblocks; ++blk) {
-      if (blk > 0) {
-        const int i = U + __builtin_ctz(blk);
-        bool add;
-        if (i + 1 < C)
-          add = ((blk >> (i + 1 - U)) & 1) != 0;
-        else
-          add = topbit_fixed ? false : q_odd;
-        flip_mem<N, 0, F>(hr, hi, M + (size_t)i * N, add ? F(1) : F(-1));
-        cpx<F> p = TreeProdT<F, 0, N>::run(hr, hi);
-        cre += (double)p.re;
-        cim += (double)p.im;
-        TB_STEP_FENCE();
-      }
+    // Static steps j = 1 .. 2^U - 1 of block blk (row ctz(j) < U flips).
+    auto static_steps = [&](int blk) {
 #pragma unroll
       for (int j = 1; j < (1 << U); ++j) {
         const int i = __builtin_ctz(j);
@@ -239,6 +227,24 @@ __device__ __forceinline__ void glynn_lane(P M, int C, uint64_t q0, uint64_t qst
         }
         TB_STEP_FENCE();
       }
+    };
+    // Block 0 is peeled so the loop body is branch-free (no phi copies of h).
+    static_steps(0);
+    for (int blk = 1; blk < nblocks; ++blk) {
+      const int i = U + __builtin_ctz(blk);
+      bool add;
+      if (i + 1 < C)
+        add = ((blk >> (i + 1 - U)) & 1) != 0;
+      else
+        add = topbit_fixed ? false : q_odd;
+      flip_mem<N, 0, F>(hr, hi, M + (size_t)i * N, add ? F(1) : F(-1));
+      {
+        cpx<F> p = TreeProdT<F, 0, N>::run(hr, hi);
+        cre += (double)p.re;
+        cim += (double)p.im;
+      }
+      TB_STEP_FENCE();
+      static_steps(blk);
     }
     acc_re += cre;
     acc_im += cim;
@@ -249,7 +255,7 @@ __device__ __forceinline__ void glynn_lane(P M, int C, uint64_t q0, uint64_t qst
 // a broadcast LDS read. Grid: batch * (wpm/4) workgroups; wave wq of matrix b covers
 // chunks q = (wq + it*wpm)*64 + lane. Strides count complex elements.
 template <int N, int U, int R0, typename F>
-__global__ __launch_bounds__(256) TB_WIDE_ATTR void glynn_wide_kernel(const cpx<F> *__restrict__ A,
+__global__ __launch_bounds__(256) TB_WIDE_ATTR(N) void glynn_wide_kernel(const cpx<F> *__restrict__ A,
                                                          int64_t mstride, int C, int wpm, int iters,
                                                          cd *__restrict__ partial) {
   __shared__ cpx<F> Ms[N * N];
@@ -329,9 +335,12 @@ __global__ void to_fp32_kernel(const cd *A, int64_t mstride, int n, int batch, f
 // ------------------------------------------------------------------------------------
 namespace tbgl {
 
-#ifndef TB_GLYNN_R0
-#define TB_GLYNN_R0 0
-#endif
+// Rows held in VGPRs by the WIDE kernel: row 0 (flipped on half of all steps) when the
+// register budget of 2 waves per SIMD allows it -- FP32 always, FP64 up to n = 24.
+template <int N, typename F>
+constexpr int glynn_reg_rows() {
+  return (sizeof(F) == 4 || N <= 24) ? 1 : 0;
+}
 
 constexpr int kWideMinN = 11;  // below this a lane owns a whole matrix
 constexpr int kMaxC = 12;      // Gray bits per lane chunk
@@ -366,7 +375,7 @@ inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, int batch, t
     const GlynnPlan p = glynn_plan(N);
     constexpr int U = 2;
     const unsigned blocks = (unsigned)batch * (unsigned)(p.wpm / 4);
-    hipLaunchKernelGGL((tb::glynn_wide_kernel<N, U, TB_GLYNN_R0, F>), dim3(blocks), dim3(256), 0, s,
+    hipLaunchKernelGGL((tb::glynn_wide_kernel<N, U, glynn_reg_rows<N, F>(), F>), dim3(blocks), dim3(256), 0, s,
                        A, mstride, p.C, p.wpm, p.iters, partial);
     int32_t rc = tbh::check_hip(hipGetLastError(), "glynn_wide_kernel");
     if (rc) return rc;

@@ -1,41 +1,54 @@
-// Binary Clifford & Clifford sweep: all k particle-deleted minors in one Gray walk.
+// Clifford & Clifford submatrices sweep: all k particle-deleted minors in one Gray walk.
 //
-// Restates BSCCRyserSubmatricesPermanentCalculator.compute_permanents
+// Computes what BSCCRyserSubmatricesPermanentCalculator.compute_permanents computes
 // (theboss/boson_sampling_utilities/permanent_calculators/bs_cc_ryser_submatrices_permanent_calculator.py:73-119,
-//  bs_submatrices_permanent_calculator_base.py:150-189) for collision-free inputs:
-//   minor_nu = (-1)^R sum_{S subset of particles, nu not in S} (-1)^|S| prod_j (sum_{mu in S} V[mu][j])
-// over k input particles (Gray bits) and R = k-1 output rows (repeated for output
-// multiplicities). With k particles of multiplicity one, every binomial factor of the
-// reference's Guan walk is 1 and its Guan code is the reflected binary Gray code.
+//  bs_submatrices_permanent_calculator_base.py:150-189) for collision-free inputs: for k
+// input particles (columns V[nu][.]) and R = k-1 output rows (repeated for output
+// multiplicities), minor_nu = perm(V without particle nu).
+//
+// Formula (Glynn-type instead of the reference's Ryser walk): with half sums
+// h_j(delta) = (1/2) sum_mu delta_mu V[mu][j] and delta_0 = +1,
+//   minor_nu = sum_{delta in {+-1}^k, delta_0 = +1} (prod_mu delta_mu) delta_nu prod_j h_j(delta).
+// Expanding prod_j over row->particle maps, the delta-sum keeps exactly the maps that are
+// bijections onto the particles other than nu, i.e. perm(V without nu); fixing delta_0
+// uses the delta -> -delta symmetry of the summand. Against the reference's Ryser walk
+// this halves the Gray steps (2^(k-1) instead of 2^k) at the same per-step cost, and the
+// terms cancel far less (all partial sums have the full particle count), which keeps
+// k = 20..40 sweeps well inside 1e-10 relative error.
 //
 // Work split (same skeleton as the Glynn kernel): aligned chunks of 2^c Gray ranks, one
 // per lane; all lanes of a wave belong to one item, so the flipped particle at local
-// step l is ctz(l) for the whole wave and its column is a broadcast LDS read.
+// step l is 1 + ctz(l) for the whole wave and its column is a broadcast LDS read.
 //
-// Minors without a per-step O(k) update: keep the running chunk sum T = sum (-1)^|S| P(S)
-// and, per in-chunk bit i, a corrector E_i updated only when bit i flips
-// (E_i += 2*sigma_old*T, sigma = +1 while particle i is absent). At chunk end
-//   D_i = E_i + sigma_i T,  sum over "i absent" = (T + D_i) / 2.
-// Bits above the chunk are constant within it: the chunk adds T to minor_i iff particle i
-// is absent. Per Gray step the cost is R complex adds, R-1 complex multiplies and O(1)
-// for the minors.
+// All k minors without a per-step O(k) update: keep the running chunk sum
+// T = sum (prod delta) prod h and, per in-chunk bit b, a corrector E_b updated only when
+// bit b flips (E_b += 2 delta_old T). At chunk end the delta_{b+1}-weighted sum is
+// D_b = E_b + delta_b T. Bits above the chunk are constant in it: D_b = delta_b T.
+// Per Gray step: R complex adds, R-1 complex multiplies and O(1) for the minors.
 #include <algorithm>
 
 #include "tb_common.hpp"
 
 #ifndef TB_SWEEP_ATTR
-#define TB_SWEEP_ATTR __attribute__((amdgpu_waves_per_eu(2)))
+#define TB_SWEEP_ATTR(K) __attribute__((amdgpu_waves_per_eu((K) <= 26 ? 2 : 1)))
 #endif
 #ifndef TB_STEP_FENCE
 #define TB_STEP_FENCE() __builtin_amdgcn_sched_barrier(0)
 #endif
+// Keep particle 0's column in VGPRs for k <= TB_SWEEP_R0_MAX_K (register budget).
 #ifndef TB_SWEEP_R0
-#define TB_SWEEP_R0 0
+#define TB_SWEEP_R0 1
+#endif
+#ifndef TB_SWEEP_R0_MAX_K
+#define TB_SWEEP_R0_MAX_K 21
 #endif
 
 namespace tb {
 
-constexpr int kSweepU = 2;      // Gray bits unrolled per block (static flip rows)
+#ifndef TB_SWEEP_U
+#define TB_SWEEP_U 3
+#endif
+constexpr int kSweepU = TB_SWEEP_U;  // Gray bits unrolled per block (static flip rows)
 constexpr int kSweepMaxC = 16;  // largest chunk (bits); sizes the LDS corrector store
 
 // h += sg * column (sg = +-1 per lane, or a compile-time sign).
@@ -82,34 +95,69 @@ __device__ __forceinline__ cd sweep_prod(const double (&hr)[R], const double (&h
   return TreeProd<0, R>::run(hr, hi);
 }
 
-// One chunk of 2^c Gray ranks starting at t0 = q << c. Vs: LDS columns [k][R].
+// One chunk of 2^c Gray ranks starting at t0 = q << c. Vs: LDS columns [k][R] (particle
+// major).
 // Ed: this lane's LDS corrector slots for bits U..c-1 (stride 64 cd).
-// Returns the chunk's contribution to all K minors in mi[] (before the (-1)^R factor).
-template <int K, int U>
-__device__ __forceinline__ void sweep_chunk(const cd *Vs, cd *Ed, int c, uint64_t q, cd (&mi)[K]) {
-  constexpr int R = K - 1;
+// Returns the chunk's contribution to all K minors in mi[].
+// Column of particle 1 (Gray bit 0), held in VGPRs when R0 == 1: it flips on half of all
+// Gray steps, so this halves the LDS broadcast traffic.
+template <int R, int R0>
+struct Col0 {
+  double re[R0 ? R : 1], im[R0 ? R : 1];
+};
+
+template <int R, int MODE, int R0>
+__device__ __forceinline__ void sweep_flip_col(double (&hr)[R], double (&hi)[R], const cd *Vs,
+                                               const Col0<R, R0> &c0, int i, double sg) {
+  if constexpr (R0 == 1) {
+    if (i == 1) {  // particle 1 (compile-time at every static flip site)
+#pragma unroll
+      for (int j = 0; j < R; ++j) {
+        if constexpr (MODE == 1) {
+          hr[j] += c0.re[j];
+          hi[j] += c0.im[j];
+        } else if constexpr (MODE == -1) {
+          hr[j] -= c0.re[j];
+          hi[j] -= c0.im[j];
+        } else {
+          hr[j] = fma(sg, c0.re[j], hr[j]);
+          hi[j] = fma(sg, c0.im[j], hi[j]);
+        }
+      }
+      return;
+    }
+  }
+  sweep_flip<R, MODE>(hr, hi, Vs + i * R, sg);
+}
+
+template <int K, int U, int R0>
+__device__ __forceinline__ void sweep_chunk(const cd *Vs, const Col0<K - 1, R0> &c1, cd *Ed, int c,
+                                            uint64_t q, cd (&mi)[K]) {
+  constexpr int R = K - 1;   // output rows
+  constexpr int NB = K - 1;  // Gray bits: bit b <-> particle b + 1 (particle 0 fixed at +1)
   const uint64_t t0 = q << c;
   const uint64_t g0 = t0 ^ (t0 >> 1);
+  // Half sums h_j = (1/2) sum_nu delta_nu V[nu][j], delta_0 = +1, delta_{b+1} = (-1)^{g_b}.
   double hr[R], hi[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) {
-    hr[j] = 0.0;
-    hi[j] = 0.0;
+    hr[j] = 0.5 * Vs[j].re;
+    hi[j] = 0.5 * Vs[j].im;
   }
 #pragma unroll 1
-  for (int i = 0; i < K; ++i) {
-    const double b = (double)((g0 >> i) & 1ull);
-    const cd *col = Vs + i * R;
+  for (int b = 0; b < NB; ++b) {
+    const double d = ((g0 >> b) & 1ull) ? -0.5 : 0.5;
+    const cd *col = Vs + (b + 1) * R;
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-      hr[j] = fma(b, col[j].re, hr[j]);
-      hi[j] = fma(b, col[j].im, hi[j]);
+      hr[j] = fma(d, col[j].re, hr[j]);
+      hi[j] = fma(d, col[j].im, hi[j]);
     }
   }
   double Tr, Ti;
   {
     const cd p = sweep_prod<R>(hr, hi);
-    const double s0 = (t0 & 1ull) ? -1.0 : 1.0;
+    const double s0 = (t0 & 1ull) ? -1.0 : 1.0;  // prod(delta) = (-1)^popcount(gray(t)) = (-1)^t
     Tr = s0 * p.re;
     Ti = s0 * p.im;
   }
@@ -120,43 +168,32 @@ __device__ __forceinline__ void sweep_chunk(const cd *Vs, cd *Ed, int c, uint64_
     Esi[i] = 0.0;
   }
   for (int i = U; i < c; ++i) Ed[(i - U) * 64] = {0.0, 0.0};
-  // t_c: the Gray bit above the chunk is bit 0 of q (0 when the chunk is everything).
+  // The Gray bit above the chunk is bit 0 of q (0 when the chunk is the whole walk).
   const bool q_odd = (q & 1ull) != 0;
   const int nblocks = 1 << (c - U);
-  for (int blk = 0; blk < nblocks; ++blk) {
-    if (blk > 0) {
-      const int i = U + __builtin_ctz(blk);
-      const bool t_next = (i + 1 < c) ? (((blk >> (i + 1 - U)) & 1) != 0) : q_odd;
-      const double sg = t_next ? -1.0 : 1.0;  // particle i joins when t_{i+1} == 0
-      cd e = Ed[(i - U) * 64];
-      e.re = fma(2.0 * sg, Tr, e.re);
-      e.im = fma(2.0 * sg, Ti, e.im);
-      Ed[(i - U) * 64] = e;
-      sweep_flip<R, 0>(hr, hi, Vs + i * R, sg);
-      const cd p = sweep_prod<R>(hr, hi);
-      Tr += p.re;
-      Ti += p.im;
-      TB_STEP_FENCE();
-    }
+  // Flip of bit i at local step l: the new bit is 1 ^ t_{i+1}. t_{i+1} == 0: delta goes
+  // +1 -> -1 (h -= V, corrector += 2T); t_{i+1} == 1: delta goes -1 -> +1 (h += V,
+  // corrector -= 2T).
+  auto static_steps = [&](int blk) {
 #pragma unroll
     for (int j = 1; j < (1 << U); ++j) {
       const int i = __builtin_ctz(j);
       if (i + 1 < U) {
-        if ((j >> (i + 1)) & 1) {  // leaves
+        if ((j >> (i + 1)) & 1) {
           Esr[i] -= 2.0 * Tr;
           Esi[i] -= 2.0 * Ti;
-          sweep_flip<R, -1>(hr, hi, Vs + i * R, -1.0);
-        } else {  // joins
+          sweep_flip_col<R, 1, R0>(hr, hi, Vs, c1, i + 1, 1.0);
+        } else {
           Esr[i] += 2.0 * Tr;
           Esi[i] += 2.0 * Ti;
-          sweep_flip<R, 1>(hr, hi, Vs + i * R, 1.0);
+          sweep_flip_col<R, -1, R0>(hr, hi, Vs, c1, i + 1, -1.0);
         }
       } else {
         const bool t_next = (U < c) ? ((blk & 1) != 0) : q_odd;
-        const double sg = t_next ? -1.0 : 1.0;
-        Esr[i] = fma(2.0 * sg, Tr, Esr[i]);
-        Esi[i] = fma(2.0 * sg, Ti, Esi[i]);
-        sweep_flip<R, 0>(hr, hi, Vs + i * R, sg);
+        const double sg = t_next ? 1.0 : -1.0;
+        Esr[i] = fma(-2.0 * sg, Tr, Esr[i]);
+        Esi[i] = fma(-2.0 * sg, Ti, Esi[i]);
+        sweep_flip_col<R, 0, R0>(hr, hi, Vs, c1, i + 1, sg);
       }
       const cd p = sweep_prod<R>(hr, hi);
       if (j & 1) {
@@ -168,47 +205,71 @@ __device__ __forceinline__ void sweep_chunk(const cd *Vs, cd *Ed, int c, uint64_
       }
       TB_STEP_FENCE();
     }
+  };
+  // Block 0 is peeled so the loop body below is branch-free (no phi copies of h).
+  static_steps(0);
+  for (int blk = 1; blk < nblocks; ++blk) {
+    const int i = U + __builtin_ctz(blk);
+    const bool t_next = (i + 1 < c) ? (((blk >> (i + 1 - U)) & 1) != 0) : q_odd;
+    const double sg = t_next ? 1.0 : -1.0;
+    cd e = Ed[(i - U) * 64];
+    e.re = fma(-2.0 * sg, Tr, e.re);
+    e.im = fma(-2.0 * sg, Ti, e.im);
+    Ed[(i - U) * 64] = e;
+    sweep_flip<R, 0>(hr, hi, Vs + (i + 1) * R, sg);
+    {
+      const cd p = sweep_prod<R>(hr, hi);
+      Tr += p.re;
+      Ti += p.im;
+    }
+    TB_STEP_FENCE();
+    static_steps(blk);
   }
+  // minor_0 = T; minor_{b+1} = D_b = E_b + delta_b(end) T for in-chunk bits, delta_b T above.
   const uint64_t tend = t0 + ((uint64_t)1 << c) - 1;
   const uint64_t gend = tend ^ (tend >> 1);
+  mi[0] = {Tr, Ti};
 #pragma unroll
-  for (int i = 0; i < K; ++i) {
-    const bool absent = ((gend >> i) & 1ull) == 0;
-    if (i < c) {
+  for (int b = 0; b < NB; ++b) {
+    const double d = ((gend >> b) & 1ull) ? -1.0 : 1.0;
+    if (b < c) {
       double er, ei;
-      if (i < U) {
-        er = Esr[i < U ? i : 0];
-        ei = Esi[i < U ? i : 0];
+      if (b < U) {
+        er = Esr[b < U ? b : 0];
+        ei = Esi[b < U ? b : 0];
       } else {
-        const cd e = Ed[(i - U) * 64];
+        const cd e = Ed[(b - U) * 64];
         er = e.re;
         ei = e.im;
       }
-      const double sig = absent ? 1.0 : -1.0;
-      mi[i] = {0.5 * (Tr + er + sig * Tr), 0.5 * (Ti + ei + sig * Ti)};
+      mi[b + 1] = {fma(d, Tr, er), fma(d, Ti, ei)};
     } else {
-      mi[i] = absent ? cd{Tr, Ti} : cd{0.0, 0.0};
+      mi[b + 1] = {d * Tr, d * Ti};
     }
   }
 }
 
-// One wave per (item, wave-slot); 64-thread workgroups. Item b's particle columns are
-// staged into LDS. Lane q-chunks: q = (w + it*wpi)*64 + lane, it < iters.
-// partial[(slot*wpi + w)*K + nu] receives the wave's share of minor nu; item_map
-// (optional, device) maps slot -> item whose columns are read.
-// item_count (optional, device): only slots < *item_count are processed, so a captured
-// graph can run a data-dependent number of items.
-template <int K, int U>
-__global__ __launch_bounds__(64) TB_SWEEP_ATTR void sweep_kernel(
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2). Renumber
+// them so that consecutive logical blocks -- the waves of one item -- run on one XCD
+// and the item's columns are fetched into one L2 only.
+__device__ __forceinline__ int xcd_local_block(int p, int G) {
+  constexpr int kXcd = 8;
+  const int x = p % kXcd, i = p / kXcd;
+  const int g8 = G / kXcd, rem = G % kXcd;
+  return x * g8 + min(x, rem) + i;
+}
+
+template <int K, int U, int R0>
+__global__ __launch_bounds__(64) TB_SWEEP_ATTR(K) void sweep_kernel(
     const cd *__restrict__ Vt, int ldv, int64_t item_stride, const int *__restrict__ item_map,
     const int *__restrict__ item_count, int items, int c, int wpi, int iters,
     cd *__restrict__ partial) {
   constexpr int R = K - 1;
   extern __shared__ cd smem[];
   cd *Vs = smem;                 // K*R
-  cd *Ed = smem + K * R;         // (kSweepMaxC - U) * 64
+  cd *Ed = smem + K * R;         // (c - U) * 64 <= kSweepMaxC * 64
   const int lane = threadIdx.x;
-  const int gw = blockIdx.x;
+  const int gw = xcd_local_block(blockIdx.x, gridDim.x);
   const int slot = gw / wpi;
   const int w = gw - slot * wpi;
   int nitems = items;
@@ -221,13 +282,21 @@ __global__ __launch_bounds__(64) TB_SWEEP_ATTR void sweep_kernel(
     Vs[e] = src[(size_t)nu * ldv + j];
   }
   __syncthreads();
-  const uint64_t nchunks = (uint64_t)1 << (K - c);
+  Col0<R, R0> c1;
+  if constexpr (R0 == 1) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      c1.re[j] = Vs[R + j].re;
+      c1.im[j] = Vs[R + j].im;
+    }
+  }
+  const uint64_t nchunks = (uint64_t)1 << (K - 1 - c);
   cd acc = {0.0, 0.0};
   for (int it = 0; it < iters; ++it) {
     const uint64_t q = ((uint64_t)w + (uint64_t)it * wpi) * 64 + lane;
     cd mi[K];
     if (q < nchunks) {
-      sweep_chunk<K, U>(Vs, Ed + lane, c, q, mi);
+      sweep_chunk<K, U, R0>(Vs, c1, Ed + lane, c, q, mi);
     } else {
 #pragma unroll
       for (int i = 0; i < K; ++i) mi[i] = {0.0, 0.0};
@@ -245,7 +314,7 @@ __global__ __launch_bounds__(64) TB_SWEEP_ATTR void sweep_kernel(
   if (lane < K) partial[((size_t)slot * wpi + w) * K + lane] = acc;
 }
 
-// minors[slot*K + nu] = (-1)^R * sum_w partial[(slot*wpi + w)*K + nu], fixed order.
+// minors[slot*K + nu] = sum_w partial[(slot*wpi + w)*K + nu], fixed order.
 __global__ void sweep_reduce_kernel(const cd *__restrict__ partial, int K, int wpi, int items,
                                     const int *__restrict__ item_count, cd *__restrict__ minors) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -258,8 +327,7 @@ __global__ void sweep_reduce_kernel(const cd *__restrict__ partial, int K, int w
     re += v.re;
     im += v.im;
   }
-  const double sgn = ((K - 1) & 1) ? -1.0 : 1.0;
-  minors[idx] = {sgn * re, sgn * im};
+  minors[idx] = {re, im};
 }
 
 }  // namespace tb
@@ -273,15 +341,17 @@ struct SweepPlan {
   int c, wpi, iters;
 };
 
-// Default chunk: 2^min(10, k-6) ranks per lane (>= 4), so one wave holds 64 chunks and a
-// lane walks at most 1024 Gray steps between re-seeds.
+// Default chunk: 2^min(11, k-6) ranks per lane (>= 2^U), so one wave holds 64 chunks and a
+// lane walks at most 2048 Gray steps between re-seeds.
 inline SweepPlan sweep_plan(int k, int c_override) {
   SweepPlan p{};
-  int c = c_override > 0 ? c_override : std::min(10, k - 6);
-  c = std::max(c, tb::kSweepU);
-  c = std::min(c, std::min(k, tb::kSweepMaxC));
+  const int nb = k - 1;                      // Gray bits (particle 0 is fixed)
+  const int u = std::min(tb::kSweepU, nb);   // static bits of the kernel for this k
+  int c = c_override > 0 ? c_override : std::min(11, nb - 6);
+  c = std::max(c, u);
+  c = std::min(c, std::min(nb, tb::kSweepMaxC));
   p.c = c;
-  const int chunk_bits = k - c;                       // log2(chunks per item)
+  const int chunk_bits = nb - c;                      // log2(chunks per item)
   const int wave_bits = std::max(0, chunk_bits - 6);  // log2(waves at one chunk per lane)
   const int wpi_bits = std::min(wave_bits, 13);
   p.wpi = 1 << wpi_bits;
@@ -290,7 +360,7 @@ inline SweepPlan sweep_plan(int k, int c_override) {
 }
 
 inline size_t sweep_lds_bytes(int k) {
-  return sizeof(tb::cd) * ((size_t)k * (k - 1) + (size_t)(tb::kSweepMaxC - tb::kSweepU) * 64);
+  return sizeof(tb::cd) * ((size_t)k * (k - 1) + (size_t)tb::kSweepMaxC * 64);
 }
 
 template <int K>
@@ -298,7 +368,9 @@ int32_t launch_sweep_k(const tb::cd *Vt, int ldv, int64_t item_stride, const int
                        const int *item_count, int items, const SweepPlan &p, tb::cd *partial,
                        hipStream_t s) {
   const unsigned grid = (unsigned)items * (unsigned)p.wpi;
-  hipLaunchKernelGGL((tb::sweep_kernel<K, tb::kSweepU>), dim3(grid), dim3(64), sweep_lds_bytes(K), s,
+  constexpr int R0 = (K <= TB_SWEEP_R0_MAX_K) ? TB_SWEEP_R0 : 0;
+  constexpr int U = (K - 1 < tb::kSweepU) ? K - 1 : tb::kSweepU;
+  hipLaunchKernelGGL((tb::sweep_kernel<K, U, R0>), dim3(grid), dim3(64), sweep_lds_bytes(K), s,
                      Vt, ldv, item_stride, item_map, item_count, items, p.c, p.wpi, p.iters,
                      partial);
   return tbh::check_hip(hipGetLastError(), "sweep_kernel");

@@ -1,0 +1,90 @@
+"""Interferometer fitting: a Clements mesh trained by Adam on the negative
+log-likelihood of boson-sampling samples, with permanent gradients from the GPU backward
+kernel (no reference counterpart: theboss samples but does not fit interferometers).
+
+Mesh: m(m-1)/2 Mach-Zehnder units in the rectangular (Clements et al., Optica 3, 1460
+(2016)) layout; unit (theta, phi) on modes (k, k+1) acts as
+    [[e^{i phi} cos theta, -sin theta], [e^{i phi} sin theta, cos theta]].
+Output phases do not change |perm|, so they are not parametrised.
+Likelihood of output occupation t for collision-free input modes S:
+    p(t) = |perm(U[rows(t), S])|^2 / prod_j t_j!
+"""
+import math
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+from . import _device as D
+from .autograd import permanent
+
+
+def clements_layout(m: int) -> List[Tuple[int, int]]:
+    """(column, first mode) of every unit, m(m-1)/2 of them."""
+    units = []
+    for col in range(m):
+        for k in range(col % 2, m - 1, 2):
+            units.append((col, k))
+    return units
+
+
+def clements_unitary(theta, phi, m: int):
+    """Differentiable m x m unitary from the mesh parameters (torch tensors of length
+    m(m-1)/2, real)."""
+    T = D.torch()
+    U = T.eye(m, dtype=T.complex128, device=theta.device)
+    rows = [U[i] for i in range(m)]
+    for u, (_, k) in enumerate(clements_layout(m)):
+        c = T.cos(theta[u]).to(T.complex128)
+        s = T.sin(theta[u]).to(T.complex128)
+        e = T.exp(1j * phi[u].to(T.complex128))
+        a, b = rows[k], rows[k + 1]
+        rows[k] = e * c * a - s * b
+        rows[k + 1] = e * s * a + c * b
+    return T.stack(rows)
+
+
+def sample_index_tensors(samples: np.ndarray, input_modes: Sequence[int]):
+    """Row index matrix (S x n) of the effective matrices and log prod t_j! per sample."""
+    samples = np.asarray(samples, dtype=np.int64)
+    rows = np.stack([np.repeat(np.arange(samples.shape[1]), s) for s in samples])
+    logfact = np.array([sum(math.lgamma(c + 1) for c in s) for s in samples])
+    return rows, logfact
+
+
+def negative_log_likelihood(U, rows, cols, logfact):
+    """Mean NLL of the samples under U. rows: S x n long tensor, cols: n long tensor."""
+    T = D.torch()
+    A = U[rows][:, :, cols]  # S x n x n effective matrices (differentiable gather)
+    p = permanent(A)
+    logp = T.log(p.real ** 2 + p.imag ** 2) - logfact
+    return -logp.mean()
+
+
+def fit_clements(samples: np.ndarray, input_modes: Sequence[int], m: int, steps: int = 200,
+                 lr: float = 0.05, seed: int = 0, init=None):
+    """Adam on the NLL; returns (theta, phi, history of NLL values)."""
+    T = D.require_gpu()
+    dev = D.device()
+    nunits = m * (m - 1) // 2
+    g = T.Generator(device="cpu").manual_seed(seed)
+    if init is None:
+        theta = (T.rand(nunits, generator=g, dtype=T.float64) * math.pi / 2).to(dev)
+        phi = (T.rand(nunits, generator=g, dtype=T.float64) * 2 * math.pi).to(dev)
+    else:
+        theta, phi = (x.detach().clone().to(dev) for x in init)
+    theta.requires_grad_(True)
+    phi.requires_grad_(True)
+    rows, logfact = sample_index_tensors(samples, input_modes)
+    rows_t = T.from_numpy(rows).to(dev)
+    cols_t = T.tensor(list(input_modes), dtype=T.long, device=dev)
+    lf_t = T.from_numpy(logfact).to(dev)
+    opt = T.optim.Adam([theta, phi], lr=lr)
+    hist = []
+    for _ in range(steps):
+        opt.zero_grad()
+        U = clements_unitary(theta, phi, m)
+        loss = negative_log_likelihood(U, rows_t, cols_t, lf_t)
+        loss.backward()
+        opt.step()
+        hist.append(float(loss.detach().cpu()))
+    return theta.detach(), phi.detach(), hist

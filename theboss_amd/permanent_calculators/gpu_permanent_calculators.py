@@ -17,7 +17,7 @@ from numpy import complex128
 from .. import _device as D
 from .. import _lib as L
 from .. import ops
-from ..states import as_occupation, compact, effective_matrix
+from ..states import as_occupation, compact_batch, effective_matrix
 from .bs_permanent_calculator_interface import BSPermanentCalculatorInterface
 
 
@@ -129,41 +129,33 @@ class _GuanGPUCalculator(BSPermanentCalculatorBase):
 
     formula = L.TB_FORMULA_RYSER
 
-    def _launch(self, pairs) -> np.ndarray:
-        T = D.require_gpu()
+    def _launch(self, S: np.ndarray, Tm: np.ndarray) -> np.ndarray:
+        """One or two launches for stacked occupations S, Tm (B x m) on the shared matrix.
+        Items whose output side has fewer Guan codes are run transposed
+        (perm(U_ST) = perm(U^T_TS)), so the kernel always enumerates the smaller side."""
         U = np.asarray(self._matrix, dtype=np.complex128)
-        subs, out_mults, in_mults = [], [], []
-        for s, t in pairs:
-            om, ot = compact(t)
-            im, it = compact(s)
-            sub = U[np.ix_(om, im)]
-            if np.prod(ot.astype(np.float64) + 1) < np.prod(it.astype(np.float64) + 1):
-                sub, ot, it = sub.T, it, ot  # perm(U_ST) = perm(U^T_TS)
-            subs.append(sub)
-            out_mults.append(ot)
-            in_mults.append(it)
-        J = max(1, max(len(x) for x in out_mults))
-        Dd = max(1, max(len(x) for x in in_mults))
-        B = len(pairs)
-        if J > L.TB_GUAN_MAX_ROWS or Dd > L.TB_GUAN_MAX_COLS:
-            return None
-        stack = np.zeros((B * J, Dd), dtype=np.complex128)
-        out_modes = np.zeros((B, J), dtype=np.int32)
-        out_mult = np.zeros((B, J), dtype=np.int32)
-        in_modes = np.zeros((B, Dd), dtype=np.int32)
-        in_mult = np.zeros((B, Dd), dtype=np.int32)
-        for b in range(B):
-            sub = subs[b]
-            stack[b * J:b * J + sub.shape[0], :sub.shape[1]] = sub
-            out_modes[b] = b * J + np.arange(J)
-            out_mult[b, :len(out_mults[b])] = out_mults[b]
-            in_modes[b] = np.arange(Dd)
-            in_mult[b, :len(in_mults[b])] = in_mults[b]
+        B = S.shape[0]
+        res = np.empty(B, dtype=np.complex128)
+        codes_s = np.prod(S.astype(np.float64) + 1, axis=1)
+        codes_t = np.prod(Tm.astype(np.float64) + 1, axis=1)
+        flip = codes_t < codes_s
         dev = D.device()
-        res = ops.guan_perm_batch(D.to_device_c128(stack, dev), D.to_device_i32(out_modes, dev),
-                                  D.to_device_i32(out_mult, dev), D.to_device_i32(in_modes, dev),
-                                  D.to_device_i32(in_mult, dev), self.formula)
-        return res.cpu().numpy()
+        for transposed in (False, True):
+            idx = np.nonzero(flip == transposed)[0]
+            if idx.size == 0:
+                continue
+            enum_side, prod_side = (Tm[idx], S[idx]) if transposed else (S[idx], Tm[idx])
+            in_modes, in_mult = compact_batch(enum_side)
+            out_modes, out_mult = compact_batch(prod_side)
+            if in_modes.shape[1] > L.TB_GUAN_MAX_COLS or out_modes.shape[1] > L.TB_GUAN_MAX_ROWS \
+                    or np.max(np.where(transposed, codes_t[idx], codes_s[idx])) > 2.0 ** 28:
+                return None
+            M = U.T if transposed else U
+            r = ops.guan_perm_batch(D.to_device_c128(M, dev), D.to_device_i32(out_modes, dev),
+                                    D.to_device_i32(out_mult, dev), D.to_device_i32(in_modes, dev),
+                                    D.to_device_i32(in_mult, dev), self.formula)
+            res[idx] = r.cpu().numpy()
+        return res
 
     def compute_permanent(self) -> complex128:
         s, t = self._checked_states()
@@ -175,7 +167,9 @@ class _GuanGPUCalculator(BSPermanentCalculatorBase):
         pairs = [(as_occupation(s), as_occupation(t)) for s, t in zip(input_states, output_states)]
         if not pairs:
             return np.zeros(0, dtype=np.complex128)
-        res = self._launch(pairs)
+        if any(len(p[0]) != len(pairs[0][0]) or len(p[1]) != len(p[0]) for p in pairs):
+            raise AttributeError("occupation vectors of a batch must share one length")
+        res = self._launch(np.stack([p[0] for p in pairs]), np.stack([p[1] for p in pairs]))
         if res is None:
             # Too many distinct modes for the mixed-radix kernel: expand multiplicities
             # and use the Glynn kernel on the effective matrices (same permanent).

@@ -4,7 +4,7 @@ Restates the parts of theboss/boson_sampling_utilities/boson_sampling_utilities.
 permanent path needs: mode assignment <-> mode occupation (:34-78), the effective
 scattering matrix gather (:545-626) and the 2m loss expansion (:287-342).
 """
-from typing import Sequence, Tuple
+from typing import List, Sequence, Tuple
 
 import numpy as np
 
@@ -29,6 +29,18 @@ def compact(state: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
     """(modes, multiplicities) of the occupied modes, in mode order."""
     modes = np.nonzero(state)[0]
     return modes.astype(np.int32), state[modes].astype(np.int32)
+
+
+def compact_batch(states: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """Row-wise compact for a B x m stack: occupied modes first (in mode order), padded
+    with multiplicity 0. Returns (modes, multiplicities) as B x W int32, W >= 1."""
+    states = np.asarray(states)
+    nz = states > 0
+    cnt = nz.sum(axis=1)
+    W = max(1, int(cnt.max()) if cnt.size else 1)
+    order = np.argsort(~nz, axis=1, kind="stable")[:, :W]
+    mult = np.take_along_axis(states, order, axis=1) * (np.arange(W)[None, :] < cnt[:, None])
+    return order.astype(np.int32), mult.astype(np.int32)
 
 
 def effective_matrix(matrix, input_state, output_state) -> np.ndarray:
@@ -77,3 +89,48 @@ def prepare_interferometer_matrix_in_expanded_space(interferometer_matrix) -> np
     ew = np.block([[w, z], [z, eye]])
     es = np.block([[np.diag(sigma), np.diag(loss)], [np.diag(loss), np.diag(sigma)]])
     return ev @ es @ ew
+
+
+def generate_possible_states(particles_number: int, modes_number: int,
+                             losses: bool = False) -> List[Tuple[int, ...]]:
+    """All m-mode states with n particles (or <= n with losses), each particle-number
+    block in reverse lexicographic order (boson_sampling_utilities.py:81-152)."""
+    if particles_number < 0 or modes_number < 1:
+        return []
+    if particles_number == 0:
+        return [tuple([0] * modes_number)]
+    start = 0 if losses else particles_number
+    states: List[Tuple[int, ...]] = []
+    for n in range(start, particles_number + 1):
+        block: List[Tuple[int, ...]] = []
+
+        def rec(prefix, left, modes_left):
+            if modes_left == 1:
+                block.append(tuple(prefix + [left]))
+                return
+            for c in range(left, -1, -1):
+                rec(prefix + [c], left - c, modes_left - 1)
+
+        rec([], n, modes_number)
+        states.extend(sorted(block, reverse=True))
+    return states
+
+
+def generate_lossy_n_particle_input_states(initial_state: Sequence[int],
+                                           number_of_particles_left: int) -> List[Tuple[int, ...]]:
+    """Distinct inputs left after losing particles, in the first-seen order of
+    itertools.combinations over the particle list (boson_sampling_utilities.py:155-198)."""
+    import itertools
+
+    occ = as_occupation(initial_state)
+    m = len(occ)
+    if occ.sum() == 0:
+        return [tuple(int(x) for x in occ)]
+    particles = list(np.repeat(np.arange(m), occ))
+    seen, out = set(), []
+    for comb_ in itertools.combinations(range(len(particles)), number_of_particles_left):
+        state = mode_assignment_to_mode_occupation([particles[e] for e in comb_], m)
+        if state not in seen:
+            seen.add(state)
+            out.append(state)
+    return out
