@@ -337,9 +337,12 @@ namespace tbgl {
 
 // Rows held in VGPRs by the WIDE kernel: row 0 (flipped on half of all steps) when the
 // register budget of 2 waves per SIMD allows it -- FP32 always, FP64 up to n = 24.
+#ifndef TB_GLYNN_F64_R0_MAX_N
+#define TB_GLYNN_F64_R0_MAX_N 24
+#endif
 template <int N, typename F>
 constexpr int glynn_reg_rows() {
-  return (sizeof(F) == 4 || N <= 24) ? 1 : 0;
+  return (sizeof(F) == 4 || N <= TB_GLYNN_F64_R0_MAX_N) ? 1 : 0;
 }
 
 constexpr int kWideMinN = 11;  // below this a lane owns a whole matrix

@@ -267,7 +267,7 @@ __global__ __launch_bounds__(64) TB_SWEEP_ATTR(K) void sweep_kernel(
   constexpr int R = K - 1;
   extern __shared__ cd smem[];
   cd *Vs = smem;                 // K*R
-  cd *Ed = smem + K * R;         // (c - U) * 64 <= kSweepMaxC * 64
+  cd *Ed = smem + K * R;         // (c - U) * 64 corrector slots
   const int lane = threadIdx.x;
   const int gw = xcd_local_block(blockIdx.x, gridDim.x);
   const int slot = gw / wpi;
@@ -359,8 +359,10 @@ inline SweepPlan sweep_plan(int k, int c_override) {
   return p;
 }
 
-inline size_t sweep_lds_bytes(int k) {
-  return sizeof(tb::cd) * ((size_t)k * (k - 1) + (size_t)tb::kSweepMaxC * 64);
+// LDS per wave: the item's k(k-1) columns plus 64 lanes x (c - U) corrector slots.
+inline size_t sweep_lds_bytes(int k, int c) {
+  const int u = std::min(tb::kSweepU, k - 1);
+  return sizeof(tb::cd) * ((size_t)k * (k - 1) + (size_t)std::max(c - u, 0) * 64);
 }
 
 template <int K>
@@ -370,7 +372,7 @@ int32_t launch_sweep_k(const tb::cd *Vt, int ldv, int64_t item_stride, const int
   const unsigned grid = (unsigned)items * (unsigned)p.wpi;
   constexpr int R0 = (K <= TB_SWEEP_R0_MAX_K) ? TB_SWEEP_R0 : 0;
   constexpr int U = (K - 1 < tb::kSweepU) ? K - 1 : tb::kSweepU;
-  hipLaunchKernelGGL((tb::sweep_kernel<K, U, R0>), dim3(grid), dim3(64), sweep_lds_bytes(K), s,
+  hipLaunchKernelGGL((tb::sweep_kernel<K, U, R0>), dim3(grid), dim3(64), sweep_lds_bytes(K, p.c), s,
                      Vt, ldv, item_stride, item_map, item_count, items, p.c, p.wpi, p.iters,
                      partial);
   return tbh::check_hip(hipGetLastError(), "sweep_kernel");
