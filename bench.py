@@ -45,6 +45,11 @@ def main() -> int:
     ap.add_argument("--plain-samples", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="gccb",
+                    choices=["gccb", "glynn", "grad", "gcca", "lossy", "fit"],
+                    help="gccb: the benchmark line (default); the others measure the "
+                         "single-large-permanent, differentiation, GCC-A, uniform-loss and "
+                         "interferometer-fit workloads and print their own JSON line")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run the k=20 sweep twice and a 256 MiB device copy (the "
                          "calibration reference) for rocprofv3 --pmc passes")
@@ -65,6 +70,11 @@ def main() -> int:
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.workload != "gccb":
+        from theboss_amd import workloads
+
+        print(json.dumps(getattr(workloads, args.workload)(orc)))
+        return 0
     U = orc.haar_unitary(M, SEED)
     Ud = D.to_device_c128(U, dev)
     if args.pmc_probe:

@@ -14,7 +14,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t tbgl_partial_bytes(int n, int batch) {
   if (n < tbgl::kWideMinN) return 0;
-  return align256((size_t)batch * tbgl::glynn_plan(n).wpm * sizeof(tb::cd));
+  return align256((size_t)batch * tbgl::glynn_plan(n, batch).wpm * sizeof(tb::cd));
 }
 }  // namespace
 

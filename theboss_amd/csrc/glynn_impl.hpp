@@ -353,11 +353,20 @@ struct GlynnPlan {
   int C, wpm, iters;
 };
 
-// WIDE plan: chunks of 2^C ranks, at least 4 waves (one workgroup) per matrix.
-inline GlynnPlan glynn_plan(int n) {
+// WIDE plan: chunks of 2^C ranks. C shrinks (down to 7) until the batch provides at
+// least ~8192 waves (4 rounds of 2 waves on every SIMD), so small batches of mid-size
+// matrices still fill the GPU; C <= 12 bounds a lane's walk between re-seeds, and a
+// matrix gets at least 4 waves (one workgroup). Re-seeding costs n^2 per 2^C steps.
+inline GlynnPlan glynn_plan(int n, int batch) {
   GlynnPlan p{};
   const int nb = n - 1;
-  p.C = std::min(kMaxC, nb - 8);
+  int want_bits = 0;  // log2(waves per matrix needed for ~8192 waves in total)
+  while ((batch << want_bits) < 8192 && want_bits < 30) ++want_bits;
+  int C = nb - 6 - want_bits;
+  C = std::min(C, kMaxC);
+  C = std::max(C, std::min(7, nb - 8));
+  C = std::min(C, nb - 8);
+  p.C = C;
   const int wave_bits = nb - p.C - 6;  // log2(waves needed at one chunk per lane), >= 2
   const int wpm_bits = std::min(wave_bits, kMaxWpmBits);
   p.wpm = 1 << wpm_bits;
@@ -375,7 +384,7 @@ inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, int batch, t
     (void)partial;
     return tbh::check_hip(hipGetLastError(), "glynn_narrow_kernel");
   } else {
-    const GlynnPlan p = glynn_plan(N);
+    const GlynnPlan p = glynn_plan(N, batch);
     constexpr int U = 2;
     const unsigned blocks = (unsigned)batch * (unsigned)(p.wpm / 4);
     hipLaunchKernelGGL((tb::glynn_wide_kernel<N, U, glynn_reg_rows<N, F>(), F>), dim3(blocks), dim3(256), 0, s,

@@ -75,6 +75,10 @@ class BSPermanentCalculatorBase(BSPermanentCalculatorInterface):
         (no reference counterpart; the reference loops over compute_permanent)."""
         raise NotImplementedError
 
+    def compute_permanent_batch_arrays(self, S: np.ndarray, Tm: np.ndarray) -> np.ndarray:
+        """compute_permanent_batch for pre-stacked B x m occupation arrays."""
+        return self.compute_permanent_batch(list(S), list(Tm))
+
 
 def _glynn_on_effective(matrices: List[np.ndarray], precision: int) -> np.ndarray:
     """Glynn permanents of equal-size square effective matrices (one launch)."""
@@ -169,7 +173,14 @@ class _GuanGPUCalculator(BSPermanentCalculatorBase):
             return np.zeros(0, dtype=np.complex128)
         if any(len(p[0]) != len(pairs[0][0]) or len(p[1]) != len(p[0]) for p in pairs):
             raise AttributeError("occupation vectors of a batch must share one length")
-        res = self._launch(np.stack([p[0] for p in pairs]), np.stack([p[1] for p in pairs]))
+        return self.compute_permanent_batch_arrays(np.stack([p[0] for p in pairs]),
+                                                   np.stack([p[1] for p in pairs]))
+
+    def compute_permanent_batch_arrays(self, S: np.ndarray, Tm: np.ndarray) -> np.ndarray:
+        S = np.asarray(S, dtype=np.int64)
+        Tm = np.asarray(Tm, dtype=np.int64)
+        pairs = list(zip(S, Tm))
+        res = self._launch(S, Tm)
         if res is None:
             # Too many distinct modes for the mixed-radix kernel: expand multiplicities
             # and use the Glynn kernel on the effective matrices (same permanent).

@@ -41,6 +41,7 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
         self._labeled_states: Dict[int, List[Tuple[int, ...]]] = defaultdict(list)
         self.current_key: Tuple[int, ...] = tuple()
         self.current_sample_probability = 1.0
+        self._layer_cache = {}
 
     def set_new_matrix(self, new_matrix) -> None:
         self._bs_permanent_calculator.matrix = new_matrix
@@ -51,6 +52,7 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
         self._labeled_states = defaultdict(list)
         for s in input_substates(self.input_state):
             self._labeled_states[sum(s)].append(s)
+        self._layer_cache = {}
         self.pmfs = {}
         samples = []
         while len(samples) < samples_number:
@@ -81,23 +83,27 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
         w = np.asarray(w, dtype=np.float64)
         return w / w.sum()
 
+    def _layer_constants(self, k: int):
+        """Per particle number k: the substates as an array, their normalised weights and
+        the prod s! k! normalisers (independent of the r-prefix, so computed once)."""
+        if k not in self._layer_cache:
+            substates = self._labeled_states[k]
+            S = np.asarray(substates, dtype=np.int64).reshape(len(substates), -1)
+            norm = np.array([float(np.prod([factorial(c) for c in s])) * factorial(sum(s))
+                             for s in substates])
+            self._layer_cache[k] = (S, self._weights(substates), norm)
+        return self._layer_cache[k]
+
     def _calculate_new_layer_of_pmfs(self) -> None:
         k = sum(self.r_sample) + 1
-        substates = self._labeled_states[k]
-        weights = self._weights(substates)
+        S, weights, norm = self._layer_constants(k)
         m = len(self.r_sample)
-        outputs = []
-        for o in range(m):
-            out = list(self.r_sample)
-            out[o] += 1
-            outputs.append(out)
-        self.possible_outputs[self.current_key] = outputs
-        ins = [s for _ in outputs for s in substates]
-        outs = [o for o in outputs for _ in substates]
-        perms = self._bs_permanent_calculator.compute_permanent_batch(ins, outs)
-        probs = (perms.real ** 2 + perms.imag ** 2).reshape(m, len(substates))
-        norm = np.array([float(np.prod([factorial(c) for c in s])) * factorial(sum(s))
-                         for s in substates])
+        outputs = np.tile(np.asarray(self.r_sample, dtype=np.int64), (m, 1)) + np.eye(m, dtype=np.int64)
+        self.possible_outputs[self.current_key] = [list(map(int, o)) for o in outputs]
+        ins = np.tile(S, (m, 1))                      # output-major: item = o * |S| + i
+        outs = np.repeat(outputs, S.shape[0], axis=0)
+        perms = self._bs_permanent_calculator.compute_permanent_batch_arrays(ins, outs)
+        probs = (perms.real ** 2 + perms.imag ** 2).reshape(m, S.shape[0])
         pmf = (probs / norm[None, :]) @ weights
         self.pmfs[self.current_key] = pmf
 

@@ -1,0 +1,158 @@
+"""Measured runs of the secondary workloads (single large permanents, differentiation,
+GCC-A, uniform-loss GCC-B, interferometer fitting). Each returns one JSON-able record.
+
+`checker` is the CPU oracle module passed in by the caller (bench.py); it is used only
+to produce seeded inputs and reference values, never for the measured computation.
+"""
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import _device as D
+from . import _lib as L
+from . import ops
+
+FP64_PEAK = 78.6e12
+FP32_PEAK = 157.3e12
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _events():
+    T = D.torch()
+    return T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
+
+
+def _time(fn, reps=1):
+    T = D.torch()
+    fn()
+    T.cuda.synchronize()
+    e0, e1 = _events()
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    T.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def glynn(checker):
+    """16 collision-free n x n blocks of Haar(2n) per n in {24, 28, 32}; FP64 against the
+    golden oracle values, FP32 column sums against FP64."""
+    with open(os.path.join(ROOT, "tests", "golden", "glynn_large.json")) as f:
+        gold = json.load(f)["cases"]
+    rows = []
+    for n in (24, 28, 32):
+        B = 16
+        mats = []
+        for b in range(B):
+            seed = 1000 * n + b
+            U = checker.haar_unitary(2 * n, seed)
+            r = np.sort(np.random.default_rng(seed).choice(2 * n, n, replace=False))
+            mats.append(U[r, :n])
+        A = D.to_device_c128(np.stack(mats))
+        fl = B * 8 * n * 2 ** (n - 1)
+        rec = {"n": n, "batch": B, "flops": fl}
+        for prec, name in ((L.TB_PREC_FP64, "fp64"), (L.TB_PREC_FP32_COLSUM, "fp32_colsum")):
+            out = {}
+            t = _time(lambda: out.setdefault("v", ops.glynn_batch(A, prec)))
+            vals = out["v"].cpu().numpy()
+            rec[name] = {"ms": t * 1e3, "tflops": fl / t / 1e12,
+                         "frac_fp64_peak": fl / t / FP64_PEAK}
+            if prec == L.TB_PREC_FP32_COLSUM:
+                rec[name]["frac_fp32_peak"] = fl / t / FP32_PEAK
+            ref = {c["b"]: complex(c["re"], c["im"]) for c in gold if c["n"] == n}
+            errs = [abs(vals[b] - ref[b]) / abs(ref[b]) for b in ref]
+            rec[name]["max_rel_err_vs_oracle"] = max(errs)
+            rec[name]["checked_against"] = f"{len(errs)} golden oracle values"
+        rows.append(rec)
+    return {"workload": "single large permanents (Glynn, 16 per n)", "results": rows}
+
+
+def grad(checker):
+    """64 n=20 blocks of Haar(40): forward permanents + all n^2 minors (backward)."""
+    n, B = 20, 64
+    mats = []
+    for b in range(B):
+        r = np.sort(np.random.default_rng(300 + b).choice(40, n, replace=False))
+        mats.append(checker.haar_unitary(40, 300 + b)[r, :n])
+    A = D.to_device_c128(np.stack(mats))
+    t_fwd = _time(lambda: ops.glynn_batch(A), reps=3)
+    t_bwd = _time(lambda: ops.perm_grad_batch(A), reps=3)
+    f_fwd = B * 8 * n * 2 ** (n - 1)
+    f_grad_survey = B * 2 ** (n - 1) * (2 * n * n + 20 * n)  # single-sweep all-minors count
+    f_bwd_exec = B * n * 2 ** (n - 1) * (10 * n - 12)         # n C&C sweeps (executed)
+    total = t_fwd + t_bwd
+    return {"workload": "permanent + gradient, 64 x n=20", "fwd_ms": t_fwd * 1e3,
+            "bwd_ms": t_bwd * 1e3,
+            "frac_fp64_peak_survey_count": (f_fwd + f_grad_survey) / total / FP64_PEAK,
+            "frac_fp64_peak_executed": (f_fwd + f_bwd_exec) / total / FP64_PEAK,
+            "note": "the backward runs one C&C minors sweep per deleted row (n sweeps of "
+                    "2^(n-1) terms); the survey count assumes a single sweep"}
+
+
+def gcca(checker):
+    """GCC-A, m=20, n=10 photons in modes 0..9, 200 samples, Ryser on the GPU (one
+    batched launch per layer); plus one 10x10 permanent by Ryser and Glynn."""
+    from .permanent_calculators import GPUGlynnGrayPermanentCalculator, GPURyserPermanentCalculator
+    from .simulation_strategies import GeneralizedCliffordsSimulationStrategy
+
+    m, n, S = 20, 10, 200
+    U = checker.haar_unitary(m, 1)
+    inp = [1] * n + [0] * (m - n)
+    strat = GeneralizedCliffordsSimulationStrategy(GPURyserPermanentCalculator(U),
+                                                   rng=np.random.default_rng(0))
+    t0 = time.perf_counter()
+    samples = strat.simulate(inp, S)
+    dt = time.perf_counter() - t0
+    ones = [1] * n + [0] * (m - n)
+    r = GPURyserPermanentCalculator(U, ones, ones).compute_permanent()
+    g = GPUGlynnGrayPermanentCalculator(U, ones, ones).compute_permanent()
+    return {"workload": "GCC-A m=20 n=10, 200 samples (Ryser, layer-batched)",
+            "samples_per_s": S / dt, "seconds": dt, "layers_computed": len(strat.pmfs),
+            "ryser_vs_glynn_10x10_rel": abs(r - g) / abs(g),
+            "all_samples_have_n_photons": all(sum(s_) == n for s_ in samples)}
+
+
+def lossy(checker):
+    """Uniform-loss GCC-B, m=80, n=40 photons in modes 0..39, eta=0.5, 4096 samples."""
+    T = D.torch()
+    m, n, S, eta = 80, 40, 4096, 0.5
+    U = D.to_device_c128(checker.haar_unitary(m, 5))
+    parts = D.to_device_i32(np.arange(n))
+    out = T.empty((S, m), dtype=T.int32, device=U.device)
+    ops.gccb_sample(U, parts, S, 1, eta, out=out)  # capture + warm
+    T.cuda.synchronize()
+    t = _time(lambda: ops.gccb_sample(U, parts, S, 2, eta, out=out), reps=2)
+    l = out.sum(1).cpu().numpy()
+    return {"workload": "uniform-loss GCC-B m=80 n=40 eta=0.5, 4096 samples",
+            "samples_per_s": S / t, "ms_per_batch": t * 1e3, "mean_photons": float(l.mean()),
+            "max_photons": int(l.max()), "binomial_mean": n * eta}
+
+
+def fit(checker):
+    """Clements mesh m=16 (120 units) fitted by Adam to 10000 GPU samples (n=8) drawn
+    from a seeded target unitary."""
+    from .interferometer import clements_unitary, fit_clements, negative_log_likelihood, sample_index_tensors
+
+    T = D.torch()
+    m, n, S = 16, 8, 10000
+    g = T.Generator().manual_seed(7)
+    th = (T.rand(120, generator=g, dtype=T.float64) * math.pi / 2).cuda()
+    ph = (T.rand(120, generator=g, dtype=T.float64) * 2 * math.pi).cuda()
+    target = clements_unitary(th, ph, m).contiguous()
+    out, _ = ops.gccb_sample(target, D.to_device_i32(np.arange(n)), S, 3)
+    samples = out.cpu().numpy()
+    rows, lf = sample_index_tensors(samples, list(range(n)))
+    with T.no_grad():
+        nll_target = float(negative_log_likelihood(
+            target, T.from_numpy(rows).cuda(), T.arange(n, device="cuda"), T.from_numpy(lf).cuda()))
+    t0 = time.perf_counter()
+    _, _, hist = fit_clements(samples, list(range(n)), m, steps=150, lr=0.05, seed=1)
+    T.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"workload": "Clements m=16 fit by Adam on 10000 samples (n=8)",
+            "adam_steps": len(hist), "ms_per_step": 1e3 * dt / len(hist),
+            "nll_start": hist[0], "nll_end": hist[-1], "nll_of_target": nll_target}
