@@ -249,16 +249,12 @@ __device__ __forceinline__ void sweep_chunk(const cd *Vs, const Col0<K - 1, R0> 
   }
 }
 
-// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2). Renumber
-// them so that consecutive logical blocks -- the waves of one item -- run on one XCD
-// and the item's columns are fetched into one L2 only.
-__device__ __forceinline__ int xcd_local_block(int p, int G) {
-  constexpr int kXcd = 8;
-  const int x = p % kXcd, i = p / kXcd;
-  const int g8 = G / kXcd, rem = G % kXcd;
-  return x * g8 + min(x, rem) + i;
-}
-
+// Persistent grid: G one-wave blocks loop over the tasks (slot, w), slot < active
+// items, w < wpi. Blocks are dispatched round-robin over the 8 XCDs (each with its own
+// L2); XCD x takes the contiguous task range [x T/8, (x+1) T/8) so that the waves of one
+// item run on one XCD and its columns are fetched into one L2 only. The active item
+// count may come from device memory, so a captured graph launches the same grid for
+// any number of active items and idle tasks cost nothing.
 template <int K, int U, int R0>
 __global__ __launch_bounds__(64) TB_SWEEP_ATTR(K) void sweep_kernel(
     const cd *__restrict__ Vt, int ldv, int64_t item_stride, const int *__restrict__ item_map,
@@ -266,52 +262,61 @@ __global__ __launch_bounds__(64) TB_SWEEP_ATTR(K) void sweep_kernel(
     cd *__restrict__ partial) {
   constexpr int R = K - 1;
   extern __shared__ cd smem[];
-  cd *Vs = smem;                 // K*R
-  cd *Ed = smem + K * R;         // (c - U) * 64 corrector slots
+  cd *Vs = smem;          // K*R
+  cd *Ed = smem + K * R;  // (c - U) * 64 corrector slots
   const int lane = threadIdx.x;
-  const int gw = xcd_local_block(blockIdx.x, gridDim.x);
-  const int slot = gw / wpi;
-  const int w = gw - slot * wpi;
   int nitems = items;
   if (item_count) nitems = min(items, *item_count);
-  if (slot >= nitems) return;
-  const int item = item_map ? item_map[slot] : slot;
-  const cd *src = Vt + (size_t)item * item_stride;
-  for (int e = lane; e < K * R; e += 64) {
-    const int nu = e / R, j = e - nu * R;
-    Vs[e] = src[(size_t)nu * ldv + j];
-  }
-  __syncthreads();
-  Col0<R, R0> c1;
-  if constexpr (R0 == 1) {
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      c1.re[j] = Vs[R + j].re;
-      c1.im[j] = Vs[R + j].im;
-    }
-  }
+  constexpr int kXcd = 8;
+  const int G = gridDim.x, p = blockIdx.x;
+  const int x = p % kXcd, bi = p / kXcd;
+  const int nblk_x = G / kXcd + (x < G % kXcd ? 1 : 0);
+  const int64_t total = (int64_t)nitems * wpi;
+  const int64_t per_x = (total + kXcd - 1) / kXcd;
+  const int64_t lo = (int64_t)x * per_x;
+  const int64_t hi = min(total, lo + per_x);
   const uint64_t nchunks = (uint64_t)1 << (K - 1 - c);
-  cd acc = {0.0, 0.0};
-  for (int it = 0; it < iters; ++it) {
-    const uint64_t q = ((uint64_t)w + (uint64_t)it * wpi) * 64 + lane;
-    cd mi[K];
-    if (q < nchunks) {
-      sweep_chunk<K, U, R0>(Vs, c1, Ed + lane, c, q, mi);
-    } else {
-#pragma unroll
-      for (int i = 0; i < K; ++i) mi[i] = {0.0, 0.0};
+  for (int64_t task = lo + bi; task < hi; task += nblk_x) {
+    const int slot = (int)(task / wpi);
+    const int w = (int)(task - (int64_t)slot * wpi);
+    const int item = item_map ? item_map[slot] : slot;
+    const cd *src = Vt + (size_t)item * item_stride;
+    __syncthreads();  // previous task's reads of Vs are done
+    for (int e = lane; e < K * R; e += 64) {
+      const int nu = e / R, j = e - nu * R;
+      Vs[e] = src[(size_t)nu * ldv + j];
     }
+    __syncthreads();
+    Col0<R, R0> c1;
+    if constexpr (R0 == 1) {
 #pragma unroll
-    for (int i = 0; i < K; ++i) {
-      const double re = wave_sum(mi[i].re);
-      const double im = wave_sum(mi[i].im);
-      if (lane == i) {
-        acc.re += re;
-        acc.im += im;
+      for (int j = 0; j < R; ++j) {
+        c1.re[j] = Vs[R + j].re;
+        c1.im[j] = Vs[R + j].im;
       }
     }
+    cd acc = {0.0, 0.0};
+    for (int it = 0; it < iters; ++it) {
+      const uint64_t q = ((uint64_t)w + (uint64_t)it * wpi) * 64 + lane;
+      cd mi[K];
+      if (q < nchunks) {
+        sweep_chunk<K, U, R0>(Vs, c1, Ed + lane, c, q, mi);
+      } else {
+#pragma unroll
+        for (int i = 0; i < K; ++i) mi[i] = {0.0, 0.0};
+      }
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const double re = wave_sum(mi[i].re);
+        const double im = wave_sum(mi[i].im);
+        if (lane == i) {
+          acc.re += re;
+          acc.im += im;
+        }
+      }
+    }
+    if (lane < K) partial[((size_t)slot * wpi + w) * K + lane] = acc;
   }
-  if (lane < K) partial[((size_t)slot * wpi + w) * K + lane] = acc;
 }
 
 // minors[slot*K + nu] = sum_w partial[(slot*wpi + w)*K + nu], fixed order.
@@ -341,6 +346,8 @@ struct SweepPlan {
   int c, wpi, iters;
 };
 
+constexpr int64_t kSweepMaxBlocks = 8192;  // 4 one-wave blocks per SIMD of 256 CUs x 4 SIMDs x 2
+
 // Default chunk: 2^min(11, k-6) ranks per lane (>= 2^U), so one wave holds 64 chunks and a
 // lane walks at most 2048 Gray steps between re-seeds.
 inline SweepPlan sweep_plan(int k, int c_override) {
@@ -369,7 +376,9 @@ template <int K>
 int32_t launch_sweep_k(const tb::cd *Vt, int ldv, int64_t item_stride, const int *item_map,
                        const int *item_count, int items, const SweepPlan &p, tb::cd *partial,
                        hipStream_t s) {
-  const unsigned grid = (unsigned)items * (unsigned)p.wpi;
+  // Persistent grid: enough one-wave blocks to fill every SIMD a few times over.
+  const int64_t tasks = (int64_t)items * p.wpi;
+  const unsigned grid = (unsigned)std::min<int64_t>(tasks, tbs::kSweepMaxBlocks);
   constexpr int R0 = (K <= TB_SWEEP_R0_MAX_K) ? TB_SWEEP_R0 : 0;
   constexpr int U = (K - 1 < tb::kSweepU) ? K - 1 : tb::kSweepU;
   hipLaunchKernelGGL((tb::sweep_kernel<K, U, R0>), dim3(grid), dim3(64), sweep_lds_bytes(K, p.c), s,
