@@ -11,8 +11,8 @@
 // owns one chunk: it seeds its column sums at the chunk start in O(n^2) and then walks
 // the chunk with O(n) updates per term. Because chunks are aligned, the row flipped at
 // local step l is ctz(l) for every lane, so in WIDE mode (all 64 lanes of a wave on one
-// matrix) the flipped row is wave-uniform and is read with scalar loads (SGPR operands)
-// instead of 64 replicated vector loads.
+// matrix) the flipped row is wave-uniform: it is read from LDS as a broadcast, held in
+// VGPRs (row 0, when registers allow) or partly in SGPRs (row 0 otherwise).
 //
 // Half column sums h_j = s_j / 2 are carried so a flip is a plain add/sub of M[i][j];
 // perm = 2 * sum_t sign(t) prod_j h_j(t). sign(t) = (-1)^(popcount gray(t)) = (-1)^(t & 1).
@@ -27,6 +27,12 @@ namespace tb {
 // Occupancy target for the wide kernels: 2 waves per SIMD (<= 256 VGPRs).
 #ifndef TB_WIDE_ATTR
 #define TB_WIDE_ATTR(N) __attribute__((amdgpu_waves_per_eu((N) <= 40 ? 2 : 1)))
+#endif
+
+// Values of row 0 held in SGPRs by the WIDE kernels that keep no row in VGPRs (20
+// complex values = 80 SGPRs; more spills SGPRs).
+#ifndef TB_GLYNN_SROW
+#define TB_GLYNN_SROW 20
 #endif
 
 #ifndef TB_STEP_FENCE
@@ -68,24 +74,25 @@ struct TreeProdT<F, LO, LO + 1> {
 
 // MODE 1: h += row, MODE -1: h -= row, MODE 0: h += sg * row (runtime direction, one
 // FMA per component with sg = +-1 held per lane).
-template <int N, int MODE, typename F, typename P>
+// Columns J0..N-1 only (J0 > 0 when the first J0 values of the row are held in SGPRs).
+template <int N, int MODE, typename F, typename P, int J0 = 0>
 __device__ __forceinline__ void flip_mem(F (&hr)[N], F (&hi)[N], P row, F sg) {
   constexpr int G = TB_FLIP_GROUP;
-  constexpr int NG = (N + G - 1) / G;
+  constexpr int NG = (N - J0 + G - 1) / G;
   cpx<F> cur[G], nxt[G];
 #pragma unroll
   for (int e = 0; e < G; ++e)
-    if (e < N) cur[e] = row[e];
+    if (J0 + e < N) cur[e] = row[J0 + e];
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
     if (g + 1 < NG) {
 #pragma unroll
       for (int e = 0; e < G; ++e)
-        if ((g + 1) * G + e < N) nxt[e] = row[(g + 1) * G + e];
+        if (J0 + (g + 1) * G + e < N) nxt[e] = row[J0 + (g + 1) * G + e];
     }
 #pragma unroll
     for (int e = 0; e < G; ++e) {
-      const int j = g * G + e;
+      const int j = J0 + g * G + e;
       if (j < N) {
         if constexpr (MODE == 1) {
           hr[j] += cur[e].re;
@@ -130,9 +137,45 @@ struct RegRows {
   F re[R0 > 0 ? R0 : 1][N], im[R0 > 0 ? R0 : 1][N];
 };
 
-template <int N, int R0, int MODE, typename F, typename P>
+// The first S values of row 0 held in SGPRs (wave-uniform): a flip of row 0 reads them
+// as scalar operands and only the rest from LDS, which unloads the LDS return path.
+template <int S, typename F>
+struct SRow {
+  F re[S > 0 ? S : 1], im[S > 0 ? S : 1];
+};
+
+__device__ __forceinline__ double uniform_value(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float uniform_value(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
+template <int N, int R0, int S, int MODE, typename F, typename P>
 __device__ __forceinline__ void flip_row(F (&hr)[N], F (&hi)[N], const RegRows<N, R0, F> &rr,
-                                         P M, int i, F sg) {
+                                         const SRow<S, F> &sr, P M, int i, F sg) {
+  if constexpr (S > 0) {
+    if (i == 0) {
+#pragma unroll
+      for (int j = 0; j < S; ++j) {
+        if constexpr (MODE == 1) {
+          hr[j] += sr.re[j];
+          hi[j] += sr.im[j];
+        } else if constexpr (MODE == -1) {
+          hr[j] -= sr.re[j];
+          hi[j] -= sr.im[j];
+        } else {
+          hr[j] = fma(sg, sr.re[j], hr[j]);
+          hi[j] = fma(sg, sr.im[j], hi[j]);
+        }
+      }
+      flip_mem<N, MODE, F, P, S>(hr, hi, M, sg);
+      return;
+    }
+  }
   if constexpr (R0 > 0) {
     if (i < R0) {
       // i is a compile-time constant at every static flip site after unrolling.
@@ -148,11 +191,20 @@ __device__ __forceinline__ void flip_row(F (&hr)[N], F (&hi)[N], const RegRows<N
 // One lane walks `iters` chunks q = q0 + it*qstride of 2^C Gray ranks each; M is the
 // n x n matrix as complex rows. `topbit_fixed` means C == N-1 (the whole enumeration in
 // one chunk), so the Gray bit above the chunk is 0.
-template <int N, int U, int R0, typename F, typename P>
+template <int N, int U, int R0, typename F, typename P, int S = 0>
 __device__ __forceinline__ void glynn_lane(P M, int C, uint64_t q0, uint64_t qstride, int iters,
                                            double &acc_re, double &acc_im) {
   constexpr int NB = N - 1;  // Gray bits
   const bool topbit_fixed = (C == NB);
+  SRow<S, F> sr;
+  if constexpr (S > 0) {
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const cpx<F> v = M[j];
+      sr.re[j] = uniform_value(v.re);
+      sr.im[j] = uniform_value(v.im);
+    }
+  }
   RegRows<N, R0, F> rr;
   if constexpr (R0 > 0) {
 #pragma unroll
@@ -206,16 +258,16 @@ __device__ __forceinline__ void glynn_lane(P M, int C, uint64_t q0, uint64_t qst
         if (i + 1 < U) {
           // Direction fixed by the local bit i+1 of j.
           if ((j >> (i + 1)) & 1)
-            flip_row<N, R0, 1, F>(hr, hi, rr, M, i, F(1));
+            flip_row<N, R0, S, 1, F>(hr, hi, rr, sr, M, i, F(1));
           else
-            flip_row<N, R0, -1, F>(hr, hi, rr, M, i, F(-1));
+            flip_row<N, R0, S, -1, F>(hr, hi, rr, sr, M, i, F(-1));
         } else {
           bool add;
           if (U < C)
             add = (blk & 1) != 0;
           else
             add = topbit_fixed ? false : q_odd;
-          flip_row<N, R0, 0, F>(hr, hi, rr, M, i, add ? F(1) : F(-1));
+          flip_row<N, R0, S, 0, F>(hr, hi, rr, sr, M, i, add ? F(1) : F(-1));
         }
         cpx<F> p = TreeProdT<F, 0, N>::run(hr, hi);
         if (j & 1) {
@@ -267,8 +319,9 @@ __global__ __launch_bounds__(256) TB_WIDE_ATTR(N) void glynn_wide_kernel(const c
   for (int e = threadIdx.x; e < N * N; e += 256) Ms[e] = src[e];
   __syncthreads();
   double are = 0.0, aim = 0.0;
-  glynn_lane<N, U, R0, F>((const cpx<F> *)Ms, C, (uint64_t)wq * 64 + lane, (uint64_t)wpm * 64,
-                          iters, are, aim);
+  constexpr int S = (R0 == 0) ? TB_GLYNN_SROW : 0;
+  glynn_lane<N, U, R0, F, const cpx<F> *, (S < N ? S : N)>(
+      (const cpx<F> *)Ms, C, (uint64_t)wq * 64 + lane, (uint64_t)wpm * 64, iters, are, aim);
   are = wave_sum(are);
   aim = wave_sum(aim);
   if (lane == 0) partial[(size_t)b * wpm + wq] = {are, aim};
