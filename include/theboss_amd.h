@@ -135,8 +135,11 @@ int32_t tb_sweep_minors_batch(const tb_complex128 *Vt, int32_t k, int32_t ldv,
 /* ---------------------------------------------------------------------------------
  * Permanent and its gradient d perm(A) / d A[i][j] = perm(A without row i, column j)
  * for a batch of n x n matrices. No reference counterpart (the reference has no
- * differentiation); each gradient row reuses the C&C submatrices sweep with that row
- * deleted, recomputing instead of storing per-subset sums.
+ * differentiation); row i of the gradient is what the C&C submatrices calculator returns
+ * with row i deleted. For 14 <= n <= 26 one Gray sweep per group of 4 columns yields
+ * all n rows of those columns (leave-one-out products and per-row correctors); other n
+ * run one C&C minors sweep per deleted row. Both recompute instead of storing
+ * per-subset sums.
  *   perm_out   batch permanents (Laplace expansion of the gradient along row 0)
  *   grad_out   batch x n x n
  * ------------------------------------------------------------------------------- */

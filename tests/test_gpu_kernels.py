@@ -225,6 +225,42 @@ def test_grad_vs_oracle_minors(T, ops, oracle, n):
         assert np.max(np.abs(G[b] - oracle.perm_minors(As[b]))) <= 1e-12
 
 
+@pytest.mark.parametrize("n,B", [(14, 3), (15, 1), (17, 100), (19, 400), (20, 3), (21, 1), (22, 200)])
+def test_grad_single_sweep_vs_oracle_minors(T, ops, oracle, n, B):
+    """The one-sweep-per-column-group gradient (n = 14..26): every minor against the
+    oracle's double-double Glynn minors; column groups of 4 that overlap (n % 4 != 0),
+    one-item and many-item plans (chunk sizes from 2^3 to 2^11)."""
+    As = np.stack([oracle.haar_unitary(2 * n, 500 + 7 * n + b)[:n, :n] for b in range(B)])
+    perm, G = ops.perm_grad_batch(dev_c(T, As))
+    perm, G = perm.cpu().numpy(), G.cpu().numpy()
+    for b in sorted({0, B // 2, B - 1}):
+        Gref = oracle.perm_minors(As[b])
+        assert rel(G[b], Gref) <= REL, (n, b)
+        p = oracle.perm_glynn_matrix(As[b], 1)
+        assert abs(perm[b] - p) <= REL * abs(p), (n, b)
+
+
+@pytest.mark.parametrize("n", [23, 26])
+def test_grad_single_sweep_laplace_large(T, ops, oracle, n):
+    """n = 23, 26 (too slow for all n^2 oracle minors): Laplace expansion along every row
+    and every column equals the Glynn kernel's permanent, and three minors against the
+    oracle."""
+    As = np.stack([oracle.haar_unitary(2 * n, 900 + n + b)[:n, :n] for b in range(2)])
+    Ad = dev_c(T, As)
+    perm, G = ops.perm_grad_batch(Ad)
+    G = G.cpu().numpy()
+    pg = ops.glynn_batch(Ad).cpu().numpy()
+    for b in range(2):
+        rows = np.sum(As[b] * G[b], axis=1)
+        cols = np.sum(As[b] * G[b], axis=0)
+        assert np.max(np.abs(rows - pg[b])) <= REL * abs(pg[b]), (n, b)
+        assert np.max(np.abs(cols - pg[b])) <= REL * abs(pg[b]), (n, b)
+    for (i, j) in ((0, 0), (n // 2, n - 1), (n - 1, 3)):
+        minor = np.delete(np.delete(As[1], i, axis=0), j, axis=1)
+        ref = oracle.perm_glynn_matrix(minor, 1)
+        assert abs(G[1, i, j] - ref) <= REL * np.max(np.abs(G[1])), (n, i, j)
+
+
 def test_grad_n12_laplace_and_oracle(T, ops, oracle):
     n = 12
     A = oracle.haar_unitary(24, 8)[:n, :n]

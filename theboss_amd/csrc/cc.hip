@@ -10,6 +10,12 @@ int32_t run_sweep(int k, const tb::cd *Vt, int ldv, int64_t item_stride, const i
                   const int *item_count, int items, int c_override, tb::cd *partial,
                   tb::cd *minors, hipStream_t s);
 }  // namespace tbs
+namespace tbgr {
+bool grad_supported(int n);
+size_t grad_partial_bytes(int n, int batch);
+int32_t run_grad(int n, const tb::cd *A, int64_t mstride, int batch, tb::cd *partial,
+                 tb::cd *grad, hipStream_t s);
+}  // namespace tbgr
 namespace tbg {
 int32_t run_guan(const tb::cd *U, int ldu, const int *out_modes, const int *out_mult, int out_max,
                  const int *in_modes, const int *in_mult, int in_max, int batch, int formula,
@@ -191,6 +197,7 @@ extern "C" int32_t tb_cc_submatrices_batch(const tb_complex128 *U, int32_t ldu,
 
 extern "C" size_t tb_perm_grad_workspace_size(int32_t n, int32_t batch) {
   if (n < 2 || batch <= 0 || n > TB_SWEEP_MAX_K) return 0;
+  if (tbgr::grad_supported(n)) return tbgr::grad_partial_bytes(n, batch);
   const int items = batch * n;
   return align256((size_t)items * n * (n - 1) * sizeof(tb::cd)) +
          tbs::sweep_partial_bytes(n, items, 0);
@@ -219,17 +226,24 @@ extern "C" int32_t tb_perm_grad_batch(const tb_complex128 *A, int32_t n, int32_t
   if (!workspace || workspace_bytes < need)
     return tbh::fail(TB_EWORKSPACE, "grad: workspace %zu bytes < required %zu", workspace_bytes,
                      need);
-  const int items = batch * n;
-  tb::cd *Vt = (tb::cd *)workspace;
-  tb::cd *partial = (tb::cd *)((char *)workspace + align256((size_t)items * n * (n - 1) * sizeof(tb::cd)));
-  const int64_t total = (int64_t)items * n * (n - 1);
-  hipLaunchKernelGGL(tb::grad_gather_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                     (const tb::cd *)A, n, batch, matrix_stride, Vt);
-  int32_t rc = tbh::check_hip(hipGetLastError(), "grad_gather_kernel");
-  if (rc) return rc;
-  rc = tbs::run_sweep(n, Vt, n - 1, (int64_t)n * (n - 1), nullptr, nullptr, items, 0, partial,
-                      (tb::cd *)grad_out, s);
-  if (rc) return rc;
+  if (tbgr::grad_supported(n)) {  // one sweep per group of 4 columns
+    int32_t rc = tbgr::run_grad(n, (const tb::cd *)A, matrix_stride, batch, (tb::cd *)workspace,
+                                (tb::cd *)grad_out, s);
+    if (rc) return rc;
+  } else {  // one C&C minors sweep per deleted row
+    const int items = batch * n;
+    tb::cd *Vt = (tb::cd *)workspace;
+    tb::cd *partial =
+        (tb::cd *)((char *)workspace + align256((size_t)items * n * (n - 1) * sizeof(tb::cd)));
+    const int64_t total = (int64_t)items * n * (n - 1);
+    hipLaunchKernelGGL(tb::grad_gather_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       s, (const tb::cd *)A, n, batch, matrix_stride, Vt);
+    int32_t rc = tbh::check_hip(hipGetLastError(), "grad_gather_kernel");
+    if (rc) return rc;
+    rc = tbs::run_sweep(n, Vt, n - 1, (int64_t)n * (n - 1), nullptr, nullptr, items, 0, partial,
+                        (tb::cd *)grad_out, s);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(tb::grad_perm_kernel, dim3((batch + 255) / 256), dim3(256), 0, s,
                      (const tb::cd *)A, n, batch, matrix_stride, (const tb::cd *)grad_out,
                      (tb::cd *)perm_out);

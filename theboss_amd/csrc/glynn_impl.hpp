@@ -18,12 +18,10 @@
 // perm = 2 * sum_t sign(t) prod_j h_j(t). sign(t) = (-1)^(popcount gray(t)) = (-1)^(t & 1).
 #include <algorithm>
 
-#include "tb_common.hpp"
+#include "flip_common.hpp"
 
 namespace tb {
 
-// Keeps the scheduler from hoisting the next Gray step's row reads above the current
-// step (which otherwise multiplies live registers by the unroll factor and spills).
 // Occupancy target for the wide kernels: 2 waves per SIMD (<= 256 VGPRs).
 #ifndef TB_WIDE_ATTR
 #define TB_WIDE_ATTR(N) __attribute__((amdgpu_waves_per_eu((N) <= 40 ? 2 : 1)))
@@ -35,101 +33,6 @@ namespace tb {
 #define TB_GLYNN_SROW 20
 #endif
 
-#ifndef TB_STEP_FENCE
-#define TB_STEP_FENCE() __builtin_amdgcn_sched_barrier(0)
-#endif
-
-template <typename F>
-struct cpx {
-  F re, im;
-};
-
-template <typename F>
-__device__ __forceinline__ cpx<F> cmulT(cpx<F> a, cpx<F> b) {
-  cpx<F> r;
-  r.re = fma(a.re, b.re, -(a.im * b.im));
-  r.im = fma(a.re, b.im, a.im * b.re);
-  return r;
-}
-
-template <typename F, int LO, int HI>
-struct TreeProdT {
-  static __device__ __forceinline__ cpx<F> run(const F *re, const F *im) {
-    constexpr int MID = (LO + HI) / 2;
-    return cmulT<F>(TreeProdT<F, LO, MID>::run(re, im), TreeProdT<F, MID, HI>::run(re, im));
-  }
-};
-template <typename F, int LO>
-struct TreeProdT<F, LO, LO + 1> {
-  static __device__ __forceinline__ cpx<F> run(const F *re, const F *im) { return {re[LO], im[LO]}; }
-};
-
-// h += row (add) or h -= row; `row` is a wave-uniform LDS row (broadcast reads) or a
-// per-lane global row (NARROW mode). The row is consumed in groups of G complex values
-// with the reads of group g+1 issued before the adds of group g, and a scheduling
-// barrier per group, so at most 2*G loaded values are live at once.
-#ifndef TB_FLIP_GROUP
-#define TB_FLIP_GROUP 4
-#endif
-
-// MODE 1: h += row, MODE -1: h -= row, MODE 0: h += sg * row (runtime direction, one
-// FMA per component with sg = +-1 held per lane).
-// Columns J0..N-1 only (J0 > 0 when the first J0 values of the row are held in SGPRs).
-template <int N, int MODE, typename F, typename P, int J0 = 0>
-__device__ __forceinline__ void flip_mem(F (&hr)[N], F (&hi)[N], P row, F sg) {
-  constexpr int G = TB_FLIP_GROUP;
-  constexpr int NG = (N - J0 + G - 1) / G;
-  cpx<F> cur[G], nxt[G];
-#pragma unroll
-  for (int e = 0; e < G; ++e)
-    if (J0 + e < N) cur[e] = row[J0 + e];
-#pragma unroll
-  for (int g = 0; g < NG; ++g) {
-    if (g + 1 < NG) {
-#pragma unroll
-      for (int e = 0; e < G; ++e)
-        if (J0 + (g + 1) * G + e < N) nxt[e] = row[J0 + (g + 1) * G + e];
-    }
-#pragma unroll
-    for (int e = 0; e < G; ++e) {
-      const int j = J0 + g * G + e;
-      if (j < N) {
-        if constexpr (MODE == 1) {
-          hr[j] += cur[e].re;
-          hi[j] += cur[e].im;
-        } else if constexpr (MODE == -1) {
-          hr[j] -= cur[e].re;
-          hi[j] -= cur[e].im;
-        } else {
-          hr[j] = fma(sg, cur[e].re, hr[j]);
-          hi[j] = fma(sg, cur[e].im, hi[j]);
-        }
-      }
-    }
-    TB_STEP_FENCE();
-#pragma unroll
-    for (int e = 0; e < G; ++e) cur[e] = nxt[e];
-  }
-}
-
-template <int N, int MODE, typename F>
-__device__ __forceinline__ void flip_reg(F (&hr)[N], F (&hi)[N], const F (&rr)[N], const F (&ri)[N],
-                                         F sg) {
-#pragma unroll
-  for (int j = 0; j < N; ++j) {
-    if constexpr (MODE == 1) {
-      hr[j] += rr[j];
-      hi[j] += ri[j];
-    } else if constexpr (MODE == -1) {
-      hr[j] -= rr[j];
-      hi[j] -= ri[j];
-    } else {
-      hr[j] = fma(sg, rr[j], hr[j]);
-      hi[j] = fma(sg, ri[j], hi[j]);
-    }
-  }
-}
-
 // Rows 0..R0-1 (the most frequently flipped: row i flips on 2^-(i+1) of all steps) are
 // held in VGPRs; the rest are read from memory at each flip.
 template <int N, int R0, typename F>
@@ -137,42 +40,12 @@ struct RegRows {
   F re[R0 > 0 ? R0 : 1][N], im[R0 > 0 ? R0 : 1][N];
 };
 
-// The first S values of row 0 held in SGPRs (wave-uniform): a flip of row 0 reads them
-// as scalar operands and only the rest from LDS, which unloads the LDS return path.
-template <int S, typename F>
-struct SRow {
-  F re[S > 0 ? S : 1], im[S > 0 ? S : 1];
-};
-
-__device__ __forceinline__ double uniform_value(double v) {
-  const uint64_t u = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
-  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-__device__ __forceinline__ float uniform_value(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
-}
-
 template <int N, int R0, int S, int MODE, typename F, typename P>
 __device__ __forceinline__ void flip_row(F (&hr)[N], F (&hi)[N], const RegRows<N, R0, F> &rr,
                                          const SRow<S, F> &sr, P M, int i, F sg) {
   if constexpr (S > 0) {
     if (i == 0) {
-#pragma unroll
-      for (int j = 0; j < S; ++j) {
-        if constexpr (MODE == 1) {
-          hr[j] += sr.re[j];
-          hi[j] += sr.im[j];
-        } else if constexpr (MODE == -1) {
-          hr[j] -= sr.re[j];
-          hi[j] -= sr.im[j];
-        } else {
-          hr[j] = fma(sg, sr.re[j], hr[j]);
-          hi[j] = fma(sg, sr.im[j], hi[j]);
-        }
-      }
-      flip_mem<N, MODE, F, P, S>(hr, hi, M, sg);
+      flip_srow<N, S, MODE, F>(hr, hi, sr, M, sg);
       return;
     }
   }
@@ -197,14 +70,7 @@ __device__ __forceinline__ void glynn_lane(P M, int C, uint64_t q0, uint64_t qst
   constexpr int NB = N - 1;  // Gray bits
   const bool topbit_fixed = (C == NB);
   SRow<S, F> sr;
-  if constexpr (S > 0) {
-#pragma unroll
-    for (int j = 0; j < S; ++j) {
-      const cpx<F> v = M[j];
-      sr.re[j] = uniform_value(v.re);
-      sr.im[j] = uniform_value(v.im);
-    }
-  }
+  if constexpr (S > 0) load_srow(sr, M);
   RegRows<N, R0, F> rr;
   if constexpr (R0 > 0) {
 #pragma unroll

@@ -55,6 +55,48 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Wave-wide sum without LDS traffic: permlane swaps for lane distances 32 and 16 (the
+// swapped pair sums to v + v[lane ^ d] in every lane), then DPP rotations inside each
+// row of 16 lanes. Every lane receives the total.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xf, 0xf, false);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ double join64(uint32_t lo, uint32_t hi) {
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// permlane32_swap(a, b) -> {[a_lo | b_lo], [a_hi | b_hi]} (halves of 32 lanes), and
+// permlane16_swap the same on rows of 16: the two results of (v, v) add up to
+// v + v[lane ^ 32] (resp. ^ 16).
+__device__ __forceinline__ double xor32_sum(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)u, (uint32_t)u, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(u >> 32), (uint32_t)(u >> 32), false, false);
+  return join64(lo[0], hi[0]) + join64(lo[1], hi[1]);
+}
+__device__ __forceinline__ double xor16_sum(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)u, (uint32_t)u, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(u >> 32), (uint32_t)(u >> 32), false, false);
+  return join64(lo[0], hi[0]) + join64(lo[1], hi[1]);
+}
+
+__device__ __forceinline__ double wave_allsum(double v) {
+  v = xor32_sum(v);
+  v = xor16_sum(v);
+  v += dpp_mov<0x128>(v);  // row_ror:8
+  v += dpp_mov<0x124>(v);  // row_ror:4
+  v += dpp_mov<0x122>(v);  // row_ror:2
+  v += dpp_mov<0x121>(v);  // row_ror:1
+  return v;
+}
+
 __device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 }  // namespace tb
