@@ -45,6 +45,8 @@ def main() -> int:
     ap.add_argument("--plain-samples", type=int, default=24)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the built-in chunk sizes instead of tuning the sweeps first")
     ap.add_argument("--workload", default="gccb",
                     choices=["gccb", "glynn", "grad", "gcca", "lossy", "fit"],
                     help="gccb: the benchmark line (default); the others measure the "
@@ -90,6 +92,16 @@ def main() -> int:
         return 0
     particles = D.to_device_i32(np.arange(N), dev)
     out = torch.empty((SAMPLES, M), dtype=torch.int32, device=dev)
+
+    # ---- autotuner: chunk size of each step's sweep at this batch (one-time set-up)
+    tuned = None
+    if not args.no_autotune:
+        from theboss_amd import autotune
+
+        res, secs = autotune.timed(autotune.tune_sampler, M, N, SAMPLES)
+        tuned = {"seconds": secs,
+                 "chunk_bits": {k: r["chunk_bits"] for k, r in res.items()},
+                 "built_in": {k: autotune.sweep_builtin(k) for k in res}}
 
     # ---- batched path: warmup then exactly K timed steps
     seeds = [SEED + i for i in range(args.warmup + args.steps)]
@@ -178,7 +190,8 @@ def main() -> int:
         "fp64_flops_per_sample": sample_flops(N, M),
         "fp64_tflops_sampler": batched * sample_flops(N, M) / 1e12,
         "roofline": {
-            "kernel": "sweep_kernel<20,3,1> (C&C minors sweep, k=20, 4096 items)",
+            "kernel": "sweep_kernel<20,3,1> (C&C minors sweep, k=20, 4096 items, chunk 2^%d)"
+                      % (tuned["chunk_bits"].get(N, 11) if tuned else 11),
             "flops_per_launch": SAMPLES * sweep_flops(k),
             "bound": "valu",
             "achieved": achieved,
@@ -189,6 +202,7 @@ def main() -> int:
             "launch_ms": t_launch * 1e3,
         },
         "cpu_baseline": cpu,
+        "autotune": tuned,
     }
     print(json.dumps(line))
     return 0

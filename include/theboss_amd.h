@@ -63,6 +63,26 @@ const char *tb_last_error(void);
 int32_t tb_device_count(void);
 
 /* ---------------------------------------------------------------------------------
+ * Tuning table. No reference counterpart: the autotuner (theboss_amd.autotune) times
+ * candidate chunk sizes and stores the fastest here. A kernel family takes its chunk
+ * size -- log2 of the Gray ranks one lane walks between re-seeds, and with it the
+ * waves per permanent -- from this table when a call does not set one:
+ *   TB_TUNE_SWEEP  C&C minors sweep, size = k (also the steps of tb_gccb_sample)
+ *   TB_TUNE_GLYNN  tb_glynn_batch, size = n
+ *   TB_TUNE_GRAD   single-sweep gradient of tb_perm_grad_batch, size = n
+ * chunk_bits 0 selects the built-in plan; other values are clamped to the kernel's
+ * valid range when a call plans its launch. Process-wide and thread-safe; each call
+ * reads an entry once, so its workspace check and its launch agree.
+ * tb_tune_get returns the entry (>= 0) or a negative TB_E* code.
+ * ------------------------------------------------------------------------------- */
+#define TB_TUNE_SWEEP 1
+#define TB_TUNE_GLYNN 2
+#define TB_TUNE_GRAD 3
+#define TB_TUNE_MAX_SIZE 48
+int32_t tb_tune_set(int32_t kernel, int32_t size, int32_t chunk_bits);
+int32_t tb_tune_get(int32_t kernel, int32_t size);
+
+/* ---------------------------------------------------------------------------------
  * Glynn permanents of a batch of n x n matrices.
  * Replaces GlynnGrayPermanentCalculator.compute_permanent
  * (theboss/boson_sampling_utilities/permanent_calculators/glynn_gray_permanent_calculator.py:41-71)

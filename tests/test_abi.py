@@ -77,3 +77,43 @@ def test_invalid_arguments_fail_before_launch(lib, call):
 def test_zero_batch_is_a_no_op(lib):
     assert lib.tb_glynn_batch(None, 5, 0, 25, 0, None, None, 0, None) == 0
     assert lib.tb_sweep_minors_batch(None, 5, 4, 20, 0, None, None, 0, None) == 0
+
+
+def test_tuning_table_roundtrip_and_errors(lib):
+    from theboss_amd import _lib
+
+    k = 20
+    try:
+        assert lib.tb_tune_get(_lib.TB_TUNE_SWEEP, k) == 0  # built-in plan by default
+        base = lib.tb_sweep_workspace_size(k, 4096)
+        assert lib.tb_tune_set(_lib.TB_TUNE_SWEEP, k, 8) == _lib.TB_OK
+        assert lib.tb_tune_get(_lib.TB_TUNE_SWEEP, k) == 8
+        # 2^(19-8-6) waves per item instead of 2^(19-11-6): 8x the partial sums
+        assert lib.tb_sweep_workspace_size(k, 4096) == 8 * base
+        assert lib.tb_tune_set(_lib.TB_TUNE_GLYNN, 32, 10) == _lib.TB_OK
+        assert lib.tb_tune_get(_lib.TB_TUNE_GLYNN, 32) == 10
+        assert lib.tb_tune_set(_lib.TB_TUNE_GRAD, 20, 9) == _lib.TB_OK
+        assert lib.tb_tune_get(_lib.TB_TUNE_GRAD, 20) == 9
+    finally:
+        for kern, size in ((_lib.TB_TUNE_SWEEP, k), (_lib.TB_TUNE_GLYNN, 32), (_lib.TB_TUNE_GRAD, 20)):
+            lib.tb_tune_set(kern, size, 0)
+    assert lib.tb_sweep_workspace_size(k, 4096) == base
+    assert lib.tb_tune_set(7, 20, 8) == _lib.TB_EINVAL
+    assert lib.tb_tune_get(0, 20) == _lib.TB_EINVAL
+    assert lib.tb_tune_set(_lib.TB_TUNE_SWEEP, _lib.TB_TUNE_MAX_SIZE + 1, 8) == _lib.TB_ERANGE
+    assert lib.tb_tune_set(_lib.TB_TUNE_SWEEP, 20, 17) == _lib.TB_EINVAL
+    assert lib.tb_tune_set(_lib.TB_TUNE_SWEEP, 20, -1) == _lib.TB_EINVAL
+
+
+def test_sampler_workspace_follows_tuned_sweeps(lib):
+    from theboss_amd import _lib, ops
+
+    cfg = ops.gccb_config(400, 20, 4096)
+    base = lib.tb_gccb_workspace_size(ctypes.byref(cfg))
+    try:
+        lib.tb_tune_set(_lib.TB_TUNE_SWEEP, 20, 7)
+        assert lib.tb_gccb_workspace_size(ctypes.byref(cfg)) > base
+        cfg.chunk_bits[20] = 11  # an explicit per-call value wins over the table
+        assert lib.tb_gccb_workspace_size(ctypes.byref(cfg)) == base
+    finally:
+        lib.tb_tune_set(_lib.TB_TUNE_SWEEP, 20, 0)

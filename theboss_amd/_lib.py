@@ -21,6 +21,11 @@ TB_PREC_FP32_COLSUM = 1
 TB_FORMULA_RYSER = 0
 TB_FORMULA_CHIN_HUH = 1
 
+TB_TUNE_SWEEP = 1
+TB_TUNE_GLYNN = 2
+TB_TUNE_GRAD = 3
+TB_TUNE_MAX_SIZE = 48
+
 TB_GLYNN_MAX_N = 48
 TB_SWEEP_MAX_K = 40
 TB_GUAN_MAX_ROWS = 48
@@ -54,6 +59,8 @@ SIGNATURES = {
     "tb_version": (_i32, []),
     "tb_last_error": (ctypes.c_char_p, []),
     "tb_device_count": (_i32, []),
+    "tb_tune_set": (_i32, [_i32, _i32, _i32]),
+    "tb_tune_get": (_i32, [_i32, _i32]),
     "tb_glynn_workspace_size": (_sz, [_i32, _i32, _i32]),
     "tb_glynn_batch": (_i32, [_vp, _i32, _i32, _i64, _i32, _vp, _vp, _sz, _vp]),
     "tb_guan_perm_batch": (_i32, [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, _vp, _vp]),

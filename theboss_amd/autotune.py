@@ -1,0 +1,184 @@
+"""Autotuner for the chunk size of the Gray-walk kernels.
+
+Every kernel here cuts a permanent's Gray enumeration into aligned chunks of 2^c ranks,
+one chunk per lane. c sets the work per lane between O(n^2) re-seeds and per-chunk
+reductions, and, with the batch, the number of waves per permanent: larger chunks
+amortise the seeding, smaller ones give small batches enough waves and shorten the
+tail of the launch. The best c depends on the size (k or n), the batch and the machine,
+so it is measured: for each size the tuner times the kernel at candidate chunk sizes
+around the built-in plan's choice, on seeded synthetic inputs of the workload's shape,
+and stores the fastest in the library's tuning table (tb_tune_set). Every later call
+that does not set a chunk size itself uses the table, including the batched sampler,
+whose captured graphs are keyed by the resolved chunk sizes.
+
+The table is process-wide; save()/load() keep it across processes.
+"""
+import json
+import time
+from typing import Dict, Iterable, List, Optional
+
+import numpy as np
+
+from . import _device as D
+from . import _lib as L
+from . import ops
+
+KERNELS = {"sweep": L.TB_TUNE_SWEEP, "glynn": L.TB_TUNE_GLYNN, "grad": L.TB_TUNE_GRAD}
+
+
+def set_entry(kernel: str, size: int, chunk_bits: int) -> None:
+    L.check(L.load().tb_tune_set(KERNELS[kernel], int(size), int(chunk_bits)))
+
+
+def get_entry(kernel: str, size: int) -> int:
+    rc = L.load().tb_tune_get(KERNELS[kernel], int(size))
+    L.check(min(rc, 0))
+    return rc
+
+
+def reset() -> None:
+    """Back to the built-in plans for every kernel and size."""
+    for name in KERNELS:
+        for size in range(L.TB_TUNE_MAX_SIZE + 1):
+            set_entry(name, size, 0)
+
+
+def table() -> Dict[str, Dict[int, int]]:
+    """Non-zero entries of the tuning table."""
+    out: Dict[str, Dict[int, int]] = {}
+    for name in KERNELS:
+        ent = {s: get_entry(name, s) for s in range(L.TB_TUNE_MAX_SIZE + 1)}
+        out[name] = {s: c for s, c in ent.items() if c}
+    return out
+
+
+def save(path: str) -> None:
+    with open(path, "w") as f:
+        json.dump({"arch": "gfx950", "table": table()}, f, indent=1, sort_keys=True)
+
+
+def load(path: str) -> None:
+    with open(path) as f:
+        data = json.load(f)
+    for name, ent in data["table"].items():
+        for size, c in ent.items():
+            set_entry(name, int(size), int(c))
+
+
+# ------------------------------------------------------------------ built-in plans
+# Mirrors of the library's built-in chunk choices, used only to centre the candidates.
+def sweep_builtin(k: int) -> int:
+    nb = k - 1
+    return max(min(11, nb - 6), min(3, nb))
+
+
+def glynn_builtin(n: int, batch: int) -> int:
+    nb = n - 1
+    want = 0
+    while (batch << want) < 8192 and want < 30:
+        want += 1
+    c = max(nb - 6 - want, min(7, nb - 8))
+    return max(2, min(c, 12, nb - 8))
+
+
+def grad_builtin(n: int, batch: int) -> int:
+    nb = n - 1
+    groups = (n + 3) // 4
+    want = 0
+    while (batch * groups << want) < 8192 and want < 30:
+        want += 1
+    u = 4 if n <= 19 else 3
+    return min(max(min(nb - 6 - want, 11), u), nb - 7)
+
+
+def _candidates(centre: int, lo: int, hi: int, spread: int) -> List[int]:
+    return [c for c in range(centre - spread, centre + spread + 1) if lo <= c <= hi]
+
+
+def _time(fn, reps: int) -> float:
+    T = D.torch()
+    fn()  # warm (and allocate the workspace for this plan)
+    T.cuda.synchronize()
+    e0, e1 = T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    T.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / reps
+
+
+def _pick(kernel: str, size: int, cands: Iterable[int], fn, reps: int) -> dict:
+    times = {}
+    for c in cands:
+        set_entry(kernel, size, c)
+        times[c] = _time(fn, reps)
+    best = min(times, key=times.get)
+    set_entry(kernel, size, best)
+    return {"chunk_bits": best, "seconds": {c: t for c, t in sorted(times.items())}}
+
+
+def _haar_block(rows: int, cols: int, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    z = (rng.standard_normal((rows, cols)) + 1j * rng.standard_normal((rows, cols))) / np.sqrt(2)
+    q, r = np.linalg.qr(z)
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def tune_sweep(ks: Iterable[int], items: int = 4096, spread: int = 2, reps: int = 3,
+               seed: int = 0) -> Dict[int, dict]:
+    """C&C minors sweep at k particles over `items` permanents (fewer for k > 20 so a
+    launch stays short). Returns {k: {"chunk_bits", "seconds"}}."""
+    out = {}
+    for k in ks:
+        if k < 8 or k > L.TB_SWEEP_MAX_K:
+            continue  # sub-millisecond launches: the built-in plan stands
+        n_items = max(64, items >> max(0, k - 20))
+        V = _haar_block(k, k, seed + k)[: k - 1, :].T.copy()  # particle-major k x (k-1)
+        Vd = D.to_device_c128(np.ascontiguousarray(np.broadcast_to(V, (n_items, k, k - 1))))
+        cands = _candidates(sweep_builtin(k), 3, min(k - 1, 16), spread)
+        out[k] = _pick("sweep", k, cands, lambda: ops.sweep_minors_batch(Vd, k), reps)
+        out[k]["items"] = n_items
+    return out
+
+
+def tune_sampler(m: int, n: int, samples: int, **kw) -> Dict[int, dict]:
+    """The sweeps of a batched GCC-B run with `samples` samples and up to n particles."""
+    del m  # the sweep cost does not depend on the number of modes
+    return tune_sweep(range(2, n + 1), items=samples, **kw)
+
+
+def tune_glynn(ns: Iterable[int], batch: int, spread: int = 2, reps: int = 2,
+               seed: int = 0) -> Dict[int, dict]:
+    """FP64 Glynn at n over a batch of `batch` matrices."""
+    out = {}
+    for n in ns:
+        if n < 16:
+            continue
+        A = D.to_device_c128(np.stack([_haar_block(2 * n, 2 * n, seed + 31 * n + b)[:n, :n]
+                                       for b in range(batch)]))
+        cands = _candidates(glynn_builtin(n, batch), 2, min(12, n - 9), spread)
+        out[n] = _pick("glynn", n, cands, lambda: ops.glynn_batch(A), reps)
+    return out
+
+
+def tune_grad(ns: Iterable[int], batch: int, spread: int = 2, reps: int = 3,
+              seed: int = 0) -> Dict[int, dict]:
+    """Single-sweep gradient (14 <= n <= 26) over a batch of `batch` matrices."""
+    out = {}
+    for n in ns:
+        if not 14 <= n <= 26:
+            continue
+        A = D.to_device_c128(np.stack([_haar_block(2 * n, 2 * n, seed + 37 * n + b)[:n, :n]
+                                       for b in range(batch)]))
+        u = 4 if n <= 19 else 3
+        cands = _candidates(grad_builtin(n, batch), u, min(11, n - 8), spread)
+        out[n] = _pick("grad", n, cands, lambda: ops.perm_grad_batch(A), reps)
+    return out
+
+
+def timed(fn, *args, **kw):
+    """(result, wall seconds) of one tuning call."""
+    t0 = time.perf_counter()
+    r = fn(*args, **kw)
+    return r, time.perf_counter() - t0

@@ -3,10 +3,14 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <atomic>
+
 #include "tb_common.hpp"
 
 namespace {
 thread_local char g_last_error[512] = "";
+// Tuning table, zero (built-in plan) until set; static storage is zero-initialised.
+std::atomic<int32_t> g_tune[TB_TUNE_GRAD + 1][TB_TUNE_MAX_SIZE + 1];
 }
 
 namespace tbh {
@@ -24,7 +28,32 @@ int32_t check_hip(hipError_t e, const char *what) {
   return fail(TB_EHIP, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
 }
 
+int32_t tuned(int kernel, int size) {
+  if (kernel < TB_TUNE_SWEEP || kernel > TB_TUNE_GRAD || size < 0 || size > TB_TUNE_MAX_SIZE)
+    return 0;
+  return g_tune[kernel][size].load(std::memory_order_relaxed);
+}
+
 }  // namespace tbh
+
+extern "C" int32_t tb_tune_set(int32_t kernel, int32_t size, int32_t chunk_bits) {
+  if (kernel < TB_TUNE_SWEEP || kernel > TB_TUNE_GRAD)
+    return tbh::fail(TB_EINVAL, "tune: unknown kernel %d", kernel);
+  if (size < 0 || size > TB_TUNE_MAX_SIZE)
+    return tbh::fail(TB_ERANGE, "tune: size %d outside 0..%d", size, TB_TUNE_MAX_SIZE);
+  if (chunk_bits < 0 || chunk_bits > 16)
+    return tbh::fail(TB_EINVAL, "tune: chunk_bits %d outside 0..16", chunk_bits);
+  g_tune[kernel][size].store(chunk_bits, std::memory_order_relaxed);
+  return TB_OK;
+}
+
+extern "C" int32_t tb_tune_get(int32_t kernel, int32_t size) {
+  if (kernel < TB_TUNE_SWEEP || kernel > TB_TUNE_GRAD)
+    return tbh::fail(TB_EINVAL, "tune: unknown kernel %d", kernel);
+  if (size < 0 || size > TB_TUNE_MAX_SIZE)
+    return tbh::fail(TB_ERANGE, "tune: size %d outside 0..%d", size, TB_TUNE_MAX_SIZE);
+  return g_tune[kernel][size].load(std::memory_order_relaxed);
+}
 
 extern "C" int32_t tb_version(void) { return 100; /* 0.1.0 */ }
 

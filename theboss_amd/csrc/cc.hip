@@ -12,8 +12,8 @@ int32_t run_sweep(int k, const tb::cd *Vt, int ldv, int64_t item_stride, const i
 }  // namespace tbs
 namespace tbgr {
 bool grad_supported(int n);
-size_t grad_partial_bytes(int n, int batch);
-int32_t run_grad(int n, const tb::cd *A, int64_t mstride, int batch, tb::cd *partial,
+size_t grad_partial_bytes(int n, int batch, int c_pref);
+int32_t run_grad(int n, const tb::cd *A, int64_t mstride, int batch, int c_pref, tb::cd *partial,
                  tb::cd *grad, hipStream_t s);
 }  // namespace tbgr
 namespace tbg {
@@ -128,15 +128,20 @@ __global__ void grad_small_kernel(const cd *__restrict__ A, int n, int batch, in
 
 namespace {
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t cc_workspace_bytes(int k, int batch, int c) {
+  if (k < 2 || batch <= 0 || k > TB_SWEEP_MAX_K) return 0;
+  const size_t vt = align256((size_t)batch * k * (k - 1) * sizeof(tb::cd));
+  const size_t compact = align256((size_t)batch * k * sizeof(tb::cd));
+  return vt + compact + tbs::sweep_partial_bytes(k, batch, c);
 }
+}  // namespace
 
 extern "C" size_t tb_cc_workspace_size(int32_t k, int32_t out_max, int32_t in_max, int32_t batch) {
   (void)out_max;
   (void)in_max;
-  if (k < 2 || batch <= 0 || k > TB_SWEEP_MAX_K) return 0;
-  const size_t vt = align256((size_t)batch * k * (k - 1) * sizeof(tb::cd));
-  const size_t compact = align256((size_t)batch * k * sizeof(tb::cd));
-  return vt + compact + tbs::sweep_partial_bytes(k, batch, 0);
+  if (k < 2 || k > TB_SWEEP_MAX_K) return 0;
+  return cc_workspace_bytes(k, batch, tbh::tuned(TB_TUNE_SWEEP, k));
 }
 
 extern "C" int32_t tb_cc_submatrices_batch(const tb_complex128 *U, int32_t ldu,
@@ -170,7 +175,8 @@ extern "C" int32_t tb_cc_submatrices_batch(const tb_complex128 *U, int32_t ldu,
   }
   if (k > TB_SWEEP_MAX_K)
     return tbh::fail(TB_ERANGE, "cc: k=%d exceeds TB_SWEEP_MAX_K=%d", k, TB_SWEEP_MAX_K);
-  const size_t need = tb_cc_workspace_size(k, out_max, in_max, batch);
+  const int c = tbh::tuned(TB_TUNE_SWEEP, k);  // read once: size check and launch agree
+  const size_t need = cc_workspace_bytes(k, batch, c);
   if (!workspace || workspace_bytes < need)
     return tbh::fail(TB_EWORKSPACE, "cc: workspace %zu bytes < required %zu", workspace_bytes,
                      need);
@@ -186,7 +192,7 @@ extern "C" int32_t tb_cc_submatrices_batch(const tb_complex128 *U, int32_t ldu,
                      in_max, batch, k, Vt);
   int32_t rc = tbh::check_hip(hipGetLastError(), "cc_gather_kernel");
   if (rc) return rc;
-  rc = tbs::run_sweep(k, Vt, k - 1, (int64_t)k * (k - 1), nullptr, nullptr, batch, 0, partial,
+  rc = tbs::run_sweep(k, Vt, k - 1, (int64_t)k * (k - 1), nullptr, nullptr, batch, c, partial,
                       compact, s);
   if (rc) return rc;
   const int n = batch * in_max;
@@ -195,12 +201,24 @@ extern "C" int32_t tb_cc_submatrices_batch(const tb_complex128 *U, int32_t ldu,
   return tbh::check_hip(hipGetLastError(), "cc_scatter_kernel");
 }
 
-extern "C" size_t tb_perm_grad_workspace_size(int32_t n, int32_t batch) {
+namespace {
+// c: the tuning entry of the path that n takes (single-sweep gradient or n sweeps).
+int grad_tuned(int n) {
+  return tbh::tuned(tbgr::grad_supported(n) ? TB_TUNE_GRAD : TB_TUNE_SWEEP, n);
+}
+
+size_t grad_workspace_bytes(int n, int batch, int c) {
   if (n < 2 || batch <= 0 || n > TB_SWEEP_MAX_K) return 0;
-  if (tbgr::grad_supported(n)) return tbgr::grad_partial_bytes(n, batch);
+  if (tbgr::grad_supported(n)) return tbgr::grad_partial_bytes(n, batch, c);
   const int items = batch * n;
   return align256((size_t)items * n * (n - 1) * sizeof(tb::cd)) +
-         tbs::sweep_partial_bytes(n, items, 0);
+         tbs::sweep_partial_bytes(n, items, c);
+}
+}  // namespace
+
+extern "C" size_t tb_perm_grad_workspace_size(int32_t n, int32_t batch) {
+  if (n < 2 || n > TB_SWEEP_MAX_K) return 0;
+  return grad_workspace_bytes(n, batch, grad_tuned(n));
 }
 
 extern "C" int32_t tb_perm_grad_batch(const tb_complex128 *A, int32_t n, int32_t batch,
@@ -222,13 +240,14 @@ extern "C" int32_t tb_perm_grad_batch(const tb_complex128 *A, int32_t n, int32_t
                        (tb::cd *)grad_out);
     return tbh::check_hip(hipGetLastError(), "grad_small_kernel");
   }
-  const size_t need = tb_perm_grad_workspace_size(n, batch);
+  const int c = grad_tuned(n);  // read once: size check and launch agree
+  const size_t need = grad_workspace_bytes(n, batch, c);
   if (!workspace || workspace_bytes < need)
     return tbh::fail(TB_EWORKSPACE, "grad: workspace %zu bytes < required %zu", workspace_bytes,
                      need);
   if (tbgr::grad_supported(n)) {  // one sweep per group of 4 columns
-    int32_t rc = tbgr::run_grad(n, (const tb::cd *)A, matrix_stride, batch, (tb::cd *)workspace,
-                                (tb::cd *)grad_out, s);
+    int32_t rc = tbgr::run_grad(n, (const tb::cd *)A, matrix_stride, batch, c,
+                                (tb::cd *)workspace, (tb::cd *)grad_out, s);
     if (rc) return rc;
   } else {  // one C&C minors sweep per deleted row
     const int items = batch * n;
@@ -240,7 +259,7 @@ extern "C" int32_t tb_perm_grad_batch(const tb_complex128 *A, int32_t n, int32_t
                        s, (const tb::cd *)A, n, batch, matrix_stride, Vt);
     int32_t rc = tbh::check_hip(hipGetLastError(), "grad_gather_kernel");
     if (rc) return rc;
-    rc = tbs::run_sweep(n, Vt, n - 1, (int64_t)n * (n - 1), nullptr, nullptr, items, 0, partial,
+    rc = tbs::run_sweep(n, Vt, n - 1, (int64_t)n * (n - 1), nullptr, nullptr, items, c, partial,
                         (tb::cd *)grad_out, s);
     if (rc) return rc;
   }

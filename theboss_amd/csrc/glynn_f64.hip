@@ -4,25 +4,29 @@
 
 namespace tbgl {
 int32_t glynn_dispatch_f64(int n, const tb::cpx<double> *A, int64_t mstride, int batch,
-                           tb::cd *out, tb::cd *partial, hipStream_t s) {
-  return dispatch_glynn<double>(n, A, mstride, batch, out, partial, s);
+                           tb::cd *out, tb::cd *partial, int c_pref, hipStream_t s) {
+  return dispatch_glynn<double>(n, A, mstride, batch, out, partial, c_pref, s);
 }
 }  // namespace tbgl
 
 namespace {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-size_t tbgl_partial_bytes(int n, int batch) {
+size_t tbgl_partial_bytes(int n, int batch, int c_pref) {
   if (n < tbgl::kWideMinN) return 0;
-  return align256((size_t)batch * tbgl::glynn_plan(n, batch).wpm * sizeof(tb::cd));
+  return align256((size_t)batch * tbgl::glynn_plan(n, batch, c_pref).wpm * sizeof(tb::cd));
+}
+
+size_t glynn_workspace_bytes(int n, int batch, int precision, int c_pref) {
+  if (n <= 1 || batch <= 0 || n > TB_GLYNN_MAX_N) return 0;
+  size_t bytes = tbgl_partial_bytes(n, batch, c_pref);
+  if (precision == TB_PREC_FP32_COLSUM) bytes += align256((size_t)batch * n * n * sizeof(tb::cpx<float>));
+  return bytes;
 }
 }  // namespace
 
 extern "C" size_t tb_glynn_workspace_size(int32_t n, int32_t batch, int32_t precision) {
-  if (n <= 1 || batch <= 0 || n > TB_GLYNN_MAX_N) return 0;
-  size_t bytes = tbgl_partial_bytes(n, batch);
-  if (precision == TB_PREC_FP32_COLSUM) bytes += align256((size_t)batch * n * n * sizeof(tb::cpx<float>));
-  return bytes;
+  return glynn_workspace_bytes(n, batch, precision, tbh::tuned(TB_TUNE_GLYNN, n));
 }
 
 extern "C" int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t batch,
@@ -46,7 +50,8 @@ extern "C" int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t bat
   if (matrix_stride < (int64_t)n * n)
     return tbh::fail(TB_EINVAL, "glynn: matrix_stride=%lld < n*n=%d", (long long)matrix_stride,
                      n * n);
-  const size_t need = tb_glynn_workspace_size(n, batch, precision);
+  const int c_pref = tbh::tuned(TB_TUNE_GLYNN, n);  // read once: size check and launch agree
+  const size_t need = glynn_workspace_bytes(n, batch, precision, c_pref);
   if (need > 0 && (workspace == nullptr || workspace_bytes < need))
     return tbh::fail(TB_EWORKSPACE, "glynn: workspace %zu bytes < required %zu", workspace_bytes,
                      need);
@@ -59,14 +64,14 @@ extern "C" int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t bat
   tb::cd *partial = (tb::cd *)ws;
   if (precision == TB_PREC_FP64)
     return tbgl::glynn_dispatch_f64(n, (const tb::cpx<double> *)A, matrix_stride, batch, o,
-                                    partial, s);
+                                    partial, c_pref, s);
   // FP32 column sums: convert once into the tail of the workspace.
-  float *A32 = (float *)(ws + tbgl_partial_bytes(n, batch));
+  float *A32 = (float *)(ws + tbgl_partial_bytes(n, batch, c_pref));
   const int64_t total = (int64_t)batch * n * n;
   hipLaunchKernelGGL(tb::to_fp32_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
                      (const tb::cd *)A, matrix_stride, n, batch, A32);
   int32_t rc = tbh::check_hip(hipGetLastError(), "to_fp32_kernel");
   if (rc) return rc;
   return tbgl::glynn_dispatch_f32(n, (const tb::cpx<float> *)A32, (int64_t)n * n, batch, o,
-                                    partial, s);
+                                    partial, c_pref, s);
 }

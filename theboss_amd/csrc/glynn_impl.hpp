@@ -272,18 +272,25 @@ struct GlynnPlan {
   int C, wpm, iters;
 };
 
-// WIDE plan: chunks of 2^C ranks. C shrinks (down to 7) until the batch provides at
-// least ~8192 waves (4 rounds of 2 waves on every SIMD), so small batches of mid-size
-// matrices still fill the GPU; C <= 12 bounds a lane's walk between re-seeds, and a
-// matrix gets at least 4 waves (one workgroup). Re-seeding costs n^2 per 2^C steps.
-inline GlynnPlan glynn_plan(int n, int batch) {
+// WIDE plan: chunks of 2^C ranks. Built in: C shrinks (down to 7) until the batch
+// provides at least ~8192 waves (4 rounds of 2 waves on every SIMD), so small batches of
+// mid-size matrices still fill the GPU; C <= 12 bounds a lane's walk between re-seeds.
+// A tuned C (c_pref > 0) replaces that choice. Always: C >= U and a matrix gets at
+// least 4 waves (one workgroup). Re-seeding costs n^2 per 2^C steps.
+inline GlynnPlan glynn_plan(int n, int batch, int c_pref) {
   GlynnPlan p{};
   const int nb = n - 1;
-  int want_bits = 0;  // log2(waves per matrix needed for ~8192 waves in total)
-  while ((batch << want_bits) < 8192 && want_bits < 30) ++want_bits;
-  int C = nb - 6 - want_bits;
+  int C;
+  if (c_pref > 0) {
+    C = c_pref;
+  } else {
+    int want_bits = 0;  // log2(waves per matrix needed for ~8192 waves in total)
+    while (((int64_t)batch << want_bits) < 8192 && want_bits < 30) ++want_bits;
+    C = nb - 6 - want_bits;
+    C = std::max(C, std::min(7, nb - 8));
+  }
   C = std::min(C, kMaxC);
-  C = std::max(C, std::min(7, nb - 8));
+  C = std::max(C, 2);
   C = std::min(C, nb - 8);
   p.C = C;
   const int wave_bits = nb - p.C - 6;  // log2(waves needed at one chunk per lane), >= 2
@@ -295,15 +302,16 @@ inline GlynnPlan glynn_plan(int n, int batch) {
 
 template <int N, typename F>
 inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, int batch, tb::cd *out,
-                       tb::cd *partial, hipStream_t s) {
+                              tb::cd *partial, int c_pref, hipStream_t s) {
   if constexpr (N < kWideMinN) {
     constexpr int U = (N - 1) < 2 ? (N - 1) : 2;
     hipLaunchKernelGGL((tb::glynn_narrow_kernel<N, U, F>), dim3((batch + 255) / 256), dim3(256), 0,
                        s, A, mstride, batch, out);
     (void)partial;
+    (void)c_pref;
     return tbh::check_hip(hipGetLastError(), "glynn_narrow_kernel");
   } else {
-    const GlynnPlan p = glynn_plan(N, batch);
+    const GlynnPlan p = glynn_plan(N, batch, c_pref);
     constexpr int U = 2;
     const unsigned blocks = (unsigned)batch * (unsigned)(p.wpm / 4);
     hipLaunchKernelGGL((tb::glynn_wide_kernel<N, U, glynn_reg_rows<N, F>(), F>), dim3(blocks), dim3(256), 0, s,
@@ -318,18 +326,18 @@ inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, int batch, t
 
 template <typename F, int N = 2>
 inline int32_t dispatch_glynn(int n, const tb::cpx<F> *A, int64_t mstride, int batch, tb::cd *out,
-                       tb::cd *partial, hipStream_t s) {
+                              tb::cd *partial, int c_pref, hipStream_t s) {
   if constexpr (N > TB_GLYNN_MAX_N) {
     return tbh::fail(TB_ERANGE, "glynn: n=%d exceeds TB_GLYNN_MAX_N=%d", n, TB_GLYNN_MAX_N);
   } else {
-    if (n == N) return launch_glynn_n<N, F>(A, mstride, batch, out, partial, s);
-    return dispatch_glynn<F, N + 1>(n, A, mstride, batch, out, partial, s);
+    if (n == N) return launch_glynn_n<N, F>(A, mstride, batch, out, partial, c_pref, s);
+    return dispatch_glynn<F, N + 1>(n, A, mstride, batch, out, partial, c_pref, s);
   }
 }
 
 int32_t glynn_dispatch_f64(int n, const tb::cpx<double> *A, int64_t mstride, int batch,
-                           tb::cd *out, tb::cd *partial, hipStream_t s);
+                           tb::cd *out, tb::cd *partial, int c_pref, hipStream_t s);
 int32_t glynn_dispatch_f32(int n, const tb::cpx<float> *A, int64_t mstride, int batch,
-                           tb::cd *out, tb::cd *partial, hipStream_t s);
+                           tb::cd *out, tb::cd *partial, int c_pref, hipStream_t s);
 
 }  // namespace tbgl

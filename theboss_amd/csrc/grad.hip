@@ -369,17 +369,22 @@ struct GradPlan {
   int c, wpi, iters, ngroups;
 };
 
-// Chunks of 2^c ranks: c shrinks (not below U, not above 11) until the batch provides
-// ~8192 waves; the chunk index needs >= 7 bits (6 lane bits and the wave's bit 0).
-inline GradPlan grad_plan(int n, int batch) {
+// Chunks of 2^c ranks. Built in: c shrinks (not above 11) until the batch provides
+// ~8192 waves; a tuned c (c_pref > 0) replaces that choice. Always c >= U and the chunk
+// index keeps >= 7 bits (6 lane bits and the wave's bit 0).
+inline GradPlan grad_plan(int n, int batch, int c_pref) {
   GradPlan p{};
   p.ngroups = (n + tb::kGradG - 1) / tb::kGradG;
   const int nb = n - 1;
   const int64_t items = (int64_t)batch * p.ngroups;
-  int want_bits = 0;
-  while ((items << want_bits) < 8192 && want_bits < 30) ++want_bits;
-  int c = nb - 6 - want_bits;
-  c = std::min(c, 11);
+  int c;
+  if (c_pref > 0) {
+    c = c_pref;
+  } else {
+    int want_bits = 0;
+    while ((items << want_bits) < 8192 && want_bits < 30) ++want_bits;
+    c = std::min(nb - 6 - want_bits, 11);
+  }
   c = std::max(c, n <= TB_GRAD_U4_MAX_N ? 4 : 3);
   c = std::min(c, nb - 7);
   p.c = c;
@@ -394,9 +399,9 @@ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 bool grad_supported(int n) { return n >= tb::kGradMinN && n <= tb::kGradMaxN; }
 
-size_t grad_partial_bytes(int n, int batch) {
+size_t grad_partial_bytes(int n, int batch, int c_pref) {
   if (!grad_supported(n) || batch <= 0) return 0;
-  const GradPlan p = grad_plan(n, batch);
+  const GradPlan p = grad_plan(n, batch, c_pref);
   return align256((size_t)batch * p.ngroups * p.wpi * n * tb::kGradG * sizeof(tb::cd));
 }
 
@@ -415,9 +420,9 @@ int32_t dispatch_grad(int n, const tb::cd *A, int64_t mstride, int batch, const 
 }
 
 // All n^2 minors of each matrix into grad[b][i][j]; partial holds grad_partial_bytes.
-int32_t run_grad(int n, const tb::cd *A, int64_t mstride, int batch, tb::cd *partial,
+int32_t run_grad(int n, const tb::cd *A, int64_t mstride, int batch, int c_pref, tb::cd *partial,
                  tb::cd *grad, hipStream_t s) {
-  const GradPlan p = grad_plan(n, batch);
+  const GradPlan p = grad_plan(n, batch, c_pref);
   int32_t rc = dispatch_grad(n, A, mstride, batch, p, partial, s);
   if (rc) return rc;
   const int64_t total = (int64_t)batch * n * n;

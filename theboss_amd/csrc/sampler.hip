@@ -358,9 +358,17 @@ std::map<int, hipStream_t> g_capture_streams;
 
 }  // namespace
 
+// Steps whose chunk bits the caller leaves at 0 take the tuning table's entry.
+static tb_gccb_config resolve_chunks(const tb_gccb_config &cfg) {
+  tb_gccb_config c = cfg;
+  for (int k = 2; k <= std::min(c.n, TB_SWEEP_MAX_K); ++k)
+    if (c.chunk_bits[k] == 0) c.chunk_bits[k] = tbh::tuned(TB_TUNE_SWEEP, k);
+  return c;
+}
+
 extern "C" size_t tb_gccb_workspace_size(const tb_gccb_config *cfg) {
-  if (!cfg || cfg->m < 1 || cfg->n < 1 || cfg->samples < 1) return 0;
-  return layout(*cfg).total;
+  if (!cfg || cfg->m < 1 || cfg->n < 1 || cfg->samples < 1 || cfg->n > TB_SWEEP_MAX_K) return 0;
+  return layout(resolve_chunks(*cfg)).total;
 }
 
 extern "C" int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128 *U, int32_t ldu,
@@ -371,7 +379,9 @@ extern "C" int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128
                                   size_t workspace_bytes, void *stream) {
   hipStream_t s = (hipStream_t)stream;
   if (!cfg) return tbh::fail(TB_EINVAL, "gccb: cfg is NULL");
-  const tb_gccb_config c = *cfg;
+  if (cfg->n > TB_SWEEP_MAX_K)
+    return tbh::fail(TB_ERANGE, "gccb: n=%d exceeds TB_SWEEP_MAX_K=%d", cfg->n, TB_SWEEP_MAX_K);
+  const tb_gccb_config c = resolve_chunks(*cfg);  // read once: layout, graph key and launches
   if (c.m < 1 || c.n < 1 || c.samples < 1)
     return tbh::fail(TB_EINVAL, "gccb: m=%d n=%d samples=%d must be >= 1", c.m, c.n, c.samples);
   if (c.n > TB_SWEEP_MAX_K)
@@ -384,7 +394,7 @@ extern "C" int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128
   if (ldu < c.m) return tbh::fail(TB_EINVAL, "gccb: ldu=%d < m=%d", ldu, c.m);
   if (!(transmissivity >= 0.0 && transmissivity <= 1.0))
     return tbh::fail(TB_EINVAL, "gccb: transmissivity %g outside [0,1]", transmissivity);
-  const size_t need = tb_gccb_workspace_size(&c);
+  const size_t need = layout(c).total;
   if (!workspace || workspace_bytes < need)
     return tbh::fail(TB_EWORKSPACE, "gccb: workspace %zu bytes < required %zu", workspace_bytes,
                      need);

@@ -426,7 +426,7 @@ int32_t run_sweep(int k, const tb::cd *Vt, int ldv, int64_t item_stride, const i
 
 extern "C" size_t tb_sweep_workspace_size(int32_t k, int32_t batch) {
   if (k > TB_SWEEP_MAX_K) return 0;
-  return tbs::sweep_partial_bytes(k, batch, 0);
+  return tbs::sweep_partial_bytes(k, batch, tbh::tuned(TB_TUNE_SWEEP, k));
 }
 
 extern "C" int32_t tb_sweep_minors_batch(const tb_complex128 *Vt, int32_t k, int32_t ldv,
@@ -443,10 +443,11 @@ extern "C" int32_t tb_sweep_minors_batch(const tb_complex128 *Vt, int32_t k, int
   if (ldv < k - 1) return tbh::fail(TB_EINVAL, "sweep: ldv=%d < k-1=%d", ldv, k - 1);
   if (item_stride < (int64_t)k * ldv)
     return tbh::fail(TB_EINVAL, "sweep: item_stride=%lld < k*ldv", (long long)item_stride);
-  const size_t need = tb_sweep_workspace_size(k, batch);
+  const int c = tbh::tuned(TB_TUNE_SWEEP, k);  // read once: size check and launch agree
+  const size_t need = tbs::sweep_partial_bytes(k, batch, c);
   if (!workspace || workspace_bytes < need)
     return tbh::fail(TB_EWORKSPACE, "sweep: workspace %zu bytes < required %zu", workspace_bytes,
                      need);
-  return tbs::run_sweep(k, (const tb::cd *)Vt, ldv, item_stride, nullptr, nullptr, batch, 0,
+  return tbs::run_sweep(k, (const tb::cd *)Vt, ldv, item_stride, nullptr, nullptr, batch, c,
                         (tb::cd *)workspace, (tb::cd *)minors, s);
 }

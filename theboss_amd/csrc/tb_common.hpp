@@ -105,4 +105,6 @@ __device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_read
 namespace tbh {
 int32_t fail(int32_t code, const char *fmt, ...);
 int32_t check_hip(hipError_t e, const char *what);
+// Tuning-table entry (0 = built-in plan) for TB_TUNE_* kernel `kernel` at `size`.
+int32_t tuned(int kernel, int size);
 }  // namespace tbh
