@@ -103,7 +103,9 @@ def test_autotuner_fills_table_from_candidates(T, tuning):
     for k, r in res.items():
         assert r["chunk_bits"] in r["seconds"]
         assert tuning.get_entry("sweep", k) == r["chunk_bits"]
-        assert r["seconds"][r["chunk_bits"]] == min(r["seconds"].values())
+        fastest = min(r["seconds"].values())
+        # the built-in choice stands unless a candidate beats it by the margin
+        assert r["seconds"][r["chunk_bits"]] <= fastest / (1.0 - tuning.MARGIN)
     g = tuning.tune_grad([16], batch=8, reps=1)
     assert tuning.get_entry("grad", 16) == g[16]["chunk_bits"]
     assert tuning.table()["sweep"] == {k: r["chunk_bits"] for k, r in res.items()}
