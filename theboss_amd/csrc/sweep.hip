@@ -27,7 +27,7 @@
 // Per Gray step: R complex adds, R-1 complex multiplies and O(1) for the minors.
 #include <algorithm>
 
-#include "tb_common.hpp"
+#include "flip_common.hpp"
 
 #ifndef TB_SWEEP_ATTR
 #define TB_SWEEP_ATTR(K) __attribute__((amdgpu_waves_per_eu((K) <= 26 ? 2 : 1)))
@@ -35,12 +35,20 @@
 #ifndef TB_STEP_FENCE
 #define TB_STEP_FENCE() __builtin_amdgcn_sched_barrier(0)
 #endif
-// Keep particle 0's column in VGPRs for k <= TB_SWEEP_R0_MAX_K (register budget).
+// Column of particle 1 (flipped on half of all Gray steps): in VGPRs for
+// k <= TB_SWEEP_R0_MAX_K (register budget), above that its first TB_SWEEP_SROW values in
+// SGPRs (scalar operands of the adds) and the rest from LDS.
 #ifndef TB_SWEEP_R0
 #define TB_SWEEP_R0 1
 #endif
 #ifndef TB_SWEEP_R0_MAX_K
 #define TB_SWEEP_R0_MAX_K 21
+#endif
+#ifndef TB_SWEEP_R0_LARGE
+#define TB_SWEEP_R0_LARGE 2
+#endif
+#ifndef TB_SWEEP_SROW
+#define TB_SWEEP_SROW 20
 #endif
 
 namespace tb {
@@ -101,14 +109,24 @@ __device__ __forceinline__ cd sweep_prod(const double (&hr)[R], const double (&h
 // Returns the chunk's contribution to all K minors in mi[].
 // Column of particle 1 (Gray bit 0), held in VGPRs when R0 == 1: it flips on half of all
 // Gray steps, so this halves the LDS broadcast traffic.
+// R0 = 1: the whole column in VGPRs; R0 = 2: its first S values in SGPRs; R0 = 0: none.
 template <int R, int R0>
 struct Col0 {
-  double re[R0 ? R : 1], im[R0 ? R : 1];
+  static constexpr int S = (R0 == 2) ? (TB_SWEEP_SROW < R ? TB_SWEEP_SROW : R) : 0;
+  double re[R0 == 1 ? R : 1], im[R0 == 1 ? R : 1];
+  SRow<S, double> sr;
 };
 
 template <int R, int MODE, int R0>
 __device__ __forceinline__ void sweep_flip_col(double (&hr)[R], double (&hi)[R], const cd *Vs,
                                                const Col0<R, R0> &c0, int i, double sg) {
+  if constexpr (R0 == 2) {
+    if (i == 1) {  // particle 1 (compile-time at every static flip site)
+      flip_srow<R, Col0<R, R0>::S, MODE, double>(hr, hi, c0.sr,
+                                                 (const cpx<double> *)(Vs + R), sg);
+      return;
+    }
+  }
   if constexpr (R0 == 1) {
     if (i == 1) {  // particle 1 (compile-time at every static flip site)
 #pragma unroll
@@ -295,6 +313,7 @@ __global__ __launch_bounds__(64) TB_SWEEP_ATTR(K) void sweep_kernel(
         c1.im[j] = Vs[R + j].im;
       }
     }
+    if constexpr (R0 == 2) load_srow(c1.sr, (const cpx<double> *)(Vs + R));
     cd acc = {0.0, 0.0};
     for (int it = 0; it < iters; ++it) {
       const uint64_t q = ((uint64_t)w + (uint64_t)it * wpi) * 64 + lane;
@@ -379,7 +398,7 @@ int32_t launch_sweep_k(const tb::cd *Vt, int ldv, int64_t item_stride, const int
   // Persistent grid: enough one-wave blocks to fill every SIMD a few times over.
   const int64_t tasks = (int64_t)items * p.wpi;
   const unsigned grid = (unsigned)std::min<int64_t>(tasks, tbs::kSweepMaxBlocks);
-  constexpr int R0 = (K <= TB_SWEEP_R0_MAX_K) ? TB_SWEEP_R0 : 0;
+  constexpr int R0 = (K <= TB_SWEEP_R0_MAX_K) ? TB_SWEEP_R0 : TB_SWEEP_R0_LARGE;
   constexpr int U = (K - 1 < tb::kSweepU) ? K - 1 : tb::kSweepU;
   hipLaunchKernelGGL((tb::sweep_kernel<K, U, R0>), dim3(grid), dim3(64), sweep_lds_bytes(K, p.c), s,
                      Vt, ldv, item_stride, item_map, item_count, items, p.c, p.wpi, p.iters,
