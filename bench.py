@@ -5,8 +5,10 @@ photons in modes 0..19, Haar-random 400 x 400 unitary, synthetic and seeded), ru
 one replayed hipGraph. `value` is samples/s of the batched path with the interferometer
 resident in HBM. The JSON line also carries the plain path (one C&C sweep launch per
 sample step, Laplace expansion and draw on the host, the reference's per-step
-structure), the roofline of the dominant kernel (the k=20 C&C sweep, FP64 VALU) and the
-CPU baseline (the oracle's C restatement of the reference sampler on the host cores).
+structure), the roofline of the dominant kernel (the k=20 C&C sweep, FP64 VALU), the
+CPU baseline (the oracle's C restatement of the reference sampler on the host cores) and
+the metric's second part: permanent + gradient of 64 n=20 matrices (one autograd pass)
+and 16 n=32 FP64 permanents as fractions of the FP64 vector peak.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -166,6 +168,13 @@ def main() -> int:
                "sample": f"{done} GCC-B samples at n={N}, m={M} (same Haar matrix) by the "
                          f"oracle's C restatement of the reference sampler, OpenMP over samples"}
 
+    # ---- the metric's second part: permanent + gradient (64 x n=20) and single large
+    # permanents (16 x n=32) as % of the FP64 vector peak (full records under "workloads")
+    from theboss_amd import workloads
+
+    wl_grad = workloads.grad(orc)
+    wl_glynn = workloads.glynn(orc, ns=(32,), precisions=("fp64",))
+
     line = {
         "metric": METRIC,
         "value": batched,
@@ -203,6 +212,9 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "autotune": tuned,
+        "perm_grad_frac_fp64_peak": wl_grad["frac_fp64_peak"],
+        "glynn_n32_frac_fp64_peak": wl_glynn["results"][0]["fp64"]["frac_fp64_peak"],
+        "workloads": {"perm_grad_64x20": wl_grad, "glynn_16x32": wl_glynn},
     }
     print(json.dumps(line))
     return 0

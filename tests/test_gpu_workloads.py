@@ -119,6 +119,25 @@ def test_grad_rows_equal_reference_submatrices_calculator(T, oracle):
         assert np.max(np.abs(lap - perm[b])) <= REL * abs(perm[b])
 
 
+def test_autograd_forward_same_with_and_without_grad(T, oracle):
+    """With requires_grad the forward runs the gradient launch (perm from the Laplace
+    expansion of the minors); without, the Glynn kernel. Both meet the oracle bar."""
+    from theboss_amd.autograd import permanent
+
+    n = 16
+    As = np.stack([oracle.haar_unitary(2 * n, 40 + b)[:n, :n] for b in range(4)])
+    ref = oracle.perm_glynn_batch(As, 1)
+    p0 = permanent(dev_c(T, As)).cpu().numpy()
+    A = dev_c(T, As).requires_grad_(True)
+    p1 = permanent(A)
+    p1.real.sum().backward()
+    p1 = p1.detach().cpu().numpy()
+    for got in (p0, p1):
+        assert np.max(np.abs(got - ref) / np.abs(ref)) <= 1e-10
+    G = np.stack([oracle.perm_minors(a) for a in As])
+    assert np.max(np.abs(A.grad.cpu().numpy() - G.conj())) <= 1e-10 * np.max(np.abs(G))
+
+
 def test_autograd_batch_shapes(T, oracle):
     from theboss_amd.autograd import permanent
 

@@ -1,9 +1,11 @@
 """Differentiable permanent (no reference counterpart: theboss has no gradients).
 
-forward:  perm(A) for a batch of n x n complex128 matrices (Glynn kernel).
-backward: d perm / d A_ij = perm(A without row i and column j), all n^2 minors from the
-          C&C sweep kernel run once per deleted row -- recomputed in the backward pass
-          instead of storing per-subset sums from the forward pass.
+forward:  perm(A) for a batch of n x n complex128 matrices. When A requires grad, one
+          gradient launch (tb_perm_grad_batch) yields the permanents and all n^2 minors
+          d perm / d A_ij = perm(A without row i and column j); the minors (B x n x n) are
+          kept for the backward pass. Otherwise the Glynn kernel computes perm alone.
+backward: grad_out times the kept minors. Per-subset sums are recomputed inside the
+          sweep, never stored.
 The permanent is holomorphic, so in PyTorch's conjugate-Wirtinger convention the
 incoming gradient g maps to g * conj(d perm / d A).
 """
@@ -21,13 +23,15 @@ def _make_function():
     class _Permanent(T.autograd.Function):
         @staticmethod
         def forward(ctx, A):
-            ctx.save_for_backward(A)
+            if ctx.needs_input_grad[0]:
+                perm, G = ops.perm_grad_batch(A.detach())
+                ctx.save_for_backward(G)
+                return perm
             return ops.glynn_batch(A.detach())
 
         @staticmethod
         def backward(ctx, grad_out):
-            (A,) = ctx.saved_tensors
-            _, G = ops.perm_grad_batch(A.detach())
+            (G,) = ctx.saved_tensors
             return grad_out.reshape(-1, 1, 1) * G.conj()
 
     return _Permanent

@@ -38,13 +38,13 @@ def _time(fn, reps=1):
     return e0.elapsed_time(e1) / 1e3 / reps
 
 
-def glynn(checker):
+def glynn(checker, ns=(24, 28, 32), precisions=("fp64", "fp32_colsum")):
     """16 collision-free n x n blocks of Haar(2n) per n in {24, 28, 32}; FP64 against the
     golden oracle values, FP32 column sums against FP64."""
     with open(os.path.join(ROOT, "tests", "golden", "glynn_large.json")) as f:
         gold = json.load(f)["cases"]
     rows = []
-    for n in (24, 28, 32):
+    for n in ns:
         B = 16
         mats = []
         for b in range(B):
@@ -56,6 +56,8 @@ def glynn(checker):
         fl = B * 8 * n * 2 ** (n - 1)
         rec = {"n": n, "batch": B, "flops": fl}
         for prec, name in ((L.TB_PREC_FP64, "fp64"), (L.TB_PREC_FP32_COLSUM, "fp32_colsum")):
+            if name not in precisions:
+                continue
             out = {}
             t = _time(lambda: out.setdefault("v", ops.glynn_batch(A, prec)))
             vals = out["v"].cpu().numpy()
@@ -78,9 +80,20 @@ def grad(checker):
     for b in range(B):
         r = np.sort(np.random.default_rng(300 + b).choice(40, n, replace=False))
         mats.append(checker.haar_unitary(40, 300 + b)[r, :n])
+    from .autograd import permanent
+
+    T = D.torch()
     A = D.to_device_c128(np.stack(mats))
-    t_fwd = _time(lambda: ops.glynn_batch(A), reps=3)
+    t_fwd = _time(lambda: ops.glynn_batch(A), reps=3)  # forward alone (no grad)
     t_bwd = _time(lambda: ops.perm_grad_batch(A), reps=3)
+    Ag = A.clone().requires_grad_(True)
+    seed = T.ones(B, dtype=T.complex128, device=A.device)
+
+    def fwd_bwd():
+        Ag.grad = None
+        permanent(Ag).backward(seed)
+
+    t_pass = _time(fwd_bwd, reps=3)
     f_fwd = B * 8 * n * 2 ** (n - 1)
     # All n^2 minors by one sweep: 2^(n-1) (2n^2 + 20n) (n column adds, ~3n multiplies for
     # leave-one-out products, n^2 signed accumulations per step).
@@ -88,14 +101,17 @@ def grad(checker):
     # Executed: ceil(n/4) sweeps over groups of 4 columns, each step a row flip (2n), n + 3
     # complex multiplies (6 each), 4 running sums (2 each) and 4 corrector FMAs (4 each).
     f_bwd_exec = B * ((n + 3) // 4) * 2 ** (n - 1) * (8 * n + 42)
-    total = t_fwd + t_bwd
-    return {"workload": "permanent + gradient, 64 x n=20", "fwd_ms": t_fwd * 1e3,
-            "bwd_ms": t_bwd * 1e3,
-            "frac_fp64_peak": (f_fwd + f_grad) / total / FP64_PEAK,
-            "frac_fp64_peak_executed": (f_fwd + f_bwd_exec) / total / FP64_PEAK,
-            "flops_counted": f_fwd + f_grad, "flops_executed": f_fwd + f_bwd_exec,
-            "note": "backward: one Gray sweep per group of 4 columns gives all n rows of those "
-                    "columns; counted flops are the single-sweep all-minors count"}
+    return {"workload": "permanent + gradient, 64 x n=20",
+            "autograd_fwd_bwd_ms": t_pass * 1e3,
+            "frac_fp64_peak": (f_fwd + f_grad) / t_pass / FP64_PEAK,
+            "frac_fp64_peak_executed": f_bwd_exec / t_pass / FP64_PEAK,
+            "perm_only_ms": t_fwd * 1e3, "perm_and_grad_kernel_ms": t_bwd * 1e3,
+            "flops_counted": f_fwd + f_grad, "flops_executed": f_bwd_exec,
+            "note": "autograd pass: the forward of permanent() on a tensor that requires grad "
+                    "runs the gradient launch, which returns perm and all n^2 minors (one Gray "
+                    "sweep per group of 4 columns); the backward scales the kept minors. "
+                    "Counted flops: Glynn forward 8n 2^(n-1) plus the single-sweep all-minors "
+                    "count 2^(n-1)(2n^2 + 20n), per matrix"}
 
 
 def gcca(checker):
