@@ -46,6 +46,8 @@ extern "C" {
 #define TB_SWEEP_MAX_K_DOC   /* largest particle count for the binary C&C sweep */
 #define TB_GUAN_MAX_ROWS 48 /* distinct output modes for the mixed-radix kernels */
 #define TB_GUAN_MAX_COLS 32 /* distinct input modes for the mixed-radix kernels */
+#define TB_HIST_MAX_M 64    /* modes of tb_outcome_histogram */
+#define TB_HIST_MAX_N 64    /* particles of tb_outcome_histogram */
 
 typedef struct {
   double re, im;
@@ -204,6 +206,24 @@ int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128 *U, int32_
                        const int32_t *forced_steps, int32_t *out_samples,
                        tb_complex128 *trace_minors, void *workspace, size_t workspace_bytes,
                        void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * Histogram of output samples over the outcome list generate_possible_states(n, m,
+ * losses) (boson_sampling_utilities.py:81-152): one block per particle number l
+ * (ascending; only l = n when losses == 0), each in reverse lexicographic order.
+ * Replaces the counting loop of BSSampleBasedDistributionCalculator
+ * (bs_sample_based_distribution_calculator.py:92-106) with a direct rank per sample.
+ *   tb_outcome_count   number of outcomes (the length of counts), or a negative TB_E*
+ *                      code (TB_ERANGE past 2^31 - 1 outcomes or TB_HIST_MAX_M/_N)
+ *   samples            S x m int32 occupations, row stride ld
+ *   counts             uint32 per outcome, incremented (the caller zeroes them)
+ *   outside            optional uint32 counter of samples not in the list (other particle
+ *                      numbers, negative entries); NULL = not counted
+ * ------------------------------------------------------------------------------- */
+int64_t tb_outcome_count(int32_t n, int32_t m, int32_t losses);
+int32_t tb_outcome_histogram(const int32_t *samples, int32_t ld, int32_t samples_n, int32_t m,
+                             int32_t n, int32_t losses, uint32_t *counts, uint32_t *outside,
+                             void *stream);
 
 #ifdef __cplusplus
 }

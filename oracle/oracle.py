@@ -375,6 +375,30 @@ def tvd_error_bound(outcomes_number: int, samples_number: int, error_probability
     return math.sqrt(b / (2 * samples_number))
 
 
+def sample_frequencies(samples, outcomes, samples_number: int) -> np.ndarray:
+    """Outcome frequencies as the reference counts them
+    (bs_sample_based_distribution_calculator.py:92-106): each sample adds one to the first
+    outcome equal to it; samples matching none are dropped; division by samples_number."""
+    probs = np.zeros(len(outcomes), dtype=np.float64)
+    outs = [np.asarray(o) for o in outcomes]
+    for sample in samples:
+        sample = np.asarray(sample)
+        for j, o in enumerate(outs):
+            if o.shape == sample.shape and np.all(sample == o):
+                probs[j] += 1
+                break
+    return probs / samples_number
+
+
+def total_variation_distance(d1, d2) -> float:
+    """quantum_computations_utilities.py:57-78 (loop form)."""
+    assert len(d1) == len(d2)
+    t = 0.0
+    for a, b in zip(d1, d2):
+        t += abs(a - b)
+    return t / 2
+
+
 # ----------------------------------------------------------------------------------
 # GCC-A layer pmf (generalized_cliffords_simulation_strategy.py:136-247)
 # ----------------------------------------------------------------------------------

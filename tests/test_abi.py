@@ -117,3 +117,19 @@ def test_sampler_workspace_follows_tuned_sweeps(lib):
         assert lib.tb_gccb_workspace_size(ctypes.byref(cfg)) == base
     finally:
         lib.tb_tune_set(_lib.TB_TUNE_SWEEP, 20, 0)
+
+
+def test_outcome_count_host_only(lib):
+    from theboss_amd import _lib
+
+    assert lib.tb_outcome_count(3, 4, 0) == 20   # C(6, 3)
+    assert lib.tb_outcome_count(3, 4, 1) == 35   # C(7, 4): 0..3 particles
+    assert lib.tb_outcome_count(0, 5, 0) == 1
+    assert lib.tb_outcome_count(10, 20, 0) == 20030010
+    assert lib.tb_outcome_count(-1, 4, 0) == _lib.TB_EINVAL
+    assert lib.tb_outcome_count(3, 0, 0) == _lib.TB_EINVAL
+    assert lib.tb_outcome_count(3, _lib.TB_HIST_MAX_M + 1, 0) == _lib.TB_ERANGE
+    assert lib.tb_outcome_count(20, 40, 0) == _lib.TB_ERANGE  # more than 2^31 - 1
+    assert lib.tb_outcome_histogram(None, 4, 0, 4, 3, 0, None, None, None) == _lib.TB_OK
+    assert lib.tb_outcome_histogram(None, 4, 8, 4, 3, 0, None, None, None) == _lib.TB_EINVAL
+    assert lib.tb_outcome_histogram(None, 2, -1, 4, 3, 0, None, None, None) == _lib.TB_EINVAL

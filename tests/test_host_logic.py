@@ -150,3 +150,43 @@ def test_state_enumeration_matches_oracle(oracle):
     for init, left in [([1, 1, 1, 0, 0], 1), ([2, 1, 0], 2), ([1, 2, 1], 3)]:
         assert generate_lossy_n_particle_input_states(init, left) == \
             oracle.generate_lossy_n_particle_input_states(init, left)
+
+
+@pytest.mark.parametrize("n,m,losses", [(0, 3, False), (0, 2, True), (3, 4, False), (4, 5, True),
+                                        (5, 1, False), (6, 6, True), (2, 7, False), (7, 3, True)])
+def test_outcome_rank_is_the_enumeration_index(oracle, n, m, losses):
+    """The device histogram's rank (restated on the host) indexes the reference's
+    enumeration order; states outside it rank -1."""
+    from theboss_amd.states import outcome_rank
+
+    states = oracle.generate_possible_states(n, m, losses=losses)
+    assert [outcome_rank(s, n, losses) for s in states] == list(range(len(states)))
+    if m > 1:
+        assert outcome_rank([n + 1] + [0] * (m - 1), n, losses) == -1
+        if not losses and n > 0:
+            assert outcome_rank([n - 1] + [0] * (m - 1), n, losses) == -1
+
+
+def test_quantum_computations_utilities(oracle):
+    import math
+
+    from theboss_amd import quantum_computations_utilities as q
+
+    rng = np.random.default_rng(5)
+    a, b = rng.random(30), rng.random(30)
+    a, b = a / a.sum(), b / b.sum()
+    assert abs(q.count_total_variation_distance(a, b) - oracle.total_variation_distance(a, b)) < 1e-15
+    keys = [(i, 30 - i) for i in range(30)]
+    da = dict(zip(keys, a))
+    db = dict(zip(keys[:20], b[:20]))  # missing keys count as 0
+    assert abs(q.count_total_variation_distance_dicts(da, db)
+               - oracle.total_variation_distance(a, np.r_[b[:20], np.zeros(10)])) < 1e-15
+    for K, N, p in ((10, 1000, 1e-3), (126, 20000, 1e-4)):
+        assert q.count_tv_distance_error_bound_of_experiment_results(K, N, p) == oracle.tvd_error_bound(K, N, p)
+        n_min = q.compute_minimal_number_of_samples_for_desired_accuracy(K, p, 0.05)
+        assert q.count_tv_distance_error_bound_of_experiment_results(K, n_min, p) <= 0.05
+        assert q.count_tv_distance_error_bound_of_experiment_results(K, n_min - 2, p) > 0.05
+    F = q.compute_qft_matrix(7)
+    assert np.allclose(F @ F.conj().T, np.eye(7))
+    assert abs(F[1, 1] - np.exp(2j * math.pi / 7) / math.sqrt(7)) < 1e-15
+    assert q.compute_qft_matrix(0).size == 0

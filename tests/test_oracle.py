@@ -246,3 +246,26 @@ def test_gcca_layer_pmf_normalises_to_fixed_loss_marginal(oracle):
     pmf = oracle.gcca_layer_pmf(U, s, [0] * 5)
     marg = np.array([sum(s[i] * abs(U[o, i]) ** 2 for i in range(5)) for o in range(5)]) / sum(s)
     assert np.allclose(pmf / pmf.sum(), marg, atol=1e-13)
+
+
+def test_sample_frequencies_reference_semantics(oracle):
+    """First matching outcome takes the count (a repeated outcome gets 0), unmatched
+    samples are dropped, division by samples_number (known answer)."""
+    samples = [(1, 0), (0, 1), (1, 0), (2, 0)]
+    outcomes = [(1, 0), (0, 1), (1, 0)]
+    assert np.allclose(oracle.sample_frequencies(samples, outcomes, 4), [0.5, 0.25, 0.0])
+    # distinct outcomes: equal to an independent Counter-based frequency
+    import collections
+
+    rng = np.random.default_rng(3)
+    outs = [tuple(o) for o in oracle.generate_possible_states(3, 4)]
+    idx = rng.integers(0, len(outs), 500)
+    smp = [outs[i] for i in idx]
+    cnt = collections.Counter(smp)
+    assert np.allclose(oracle.sample_frequencies(smp, outs, 500), [cnt[o] / 500 for o in outs])
+
+
+def test_total_variation_distance_closed_forms(oracle):
+    assert oracle.total_variation_distance([0.2, 0.8], [0.2, 0.8]) == 0
+    assert oracle.total_variation_distance([1, 0, 0], [0, 0.5, 0.5]) == 1
+    assert abs(oracle.total_variation_distance([0.5, 0.5], [0.9, 0.1]) - 0.4) < 1e-15

@@ -30,6 +30,8 @@ TB_GLYNN_MAX_N = 48
 TB_SWEEP_MAX_K = 40
 TB_GUAN_MAX_ROWS = 48
 TB_GUAN_MAX_COLS = 32
+TB_HIST_MAX_M = 64
+TB_HIST_MAX_N = 64
 
 
 class TbError(RuntimeError):
@@ -71,6 +73,8 @@ SIGNATURES = {
     "tb_sweep_minors_batch": (_i32, [_vp, _i32, _i32, _i64, _i32, _vp, _vp, _sz, _vp]),
     "tb_perm_grad_workspace_size": (_sz, [_i32, _i32]),
     "tb_perm_grad_batch": (_i32, [_vp, _i32, _i32, _i64, _vp, _vp, _vp, _sz, _vp]),
+    "tb_outcome_count": (_i64, [_i32, _i32, _i32]),
+    "tb_outcome_histogram": (_i32, [_vp, _i32, _i32, _i32, _i32, _i32, _vp, _vp, _vp]),
     "tb_gccb_workspace_size": (_sz, [ctypes.POINTER(GccbConfig)]),
     "tb_gccb_sample": (
         _i32,

@@ -67,17 +67,24 @@ __device__ __forceinline__ double philox_uniform(uint64_t seed, uint32_t sample,
 
 enum : uint32_t { kPurposeDraw = 1, kPurposeOrder = 2, kPurposeLoss = 3 };
 
-// Per-call parameters kept in device memory so a captured graph replays unchanged.
+// Per-call parameters, the caller's buffers included, kept in device memory: a captured
+// graph depends only on the shapes and the workspace, and replays for any seed, eta,
+// interferometer, output or forced-draw buffers.
 struct GccbParams {
   uint64_t seed;
   double eta;
+  const cd *U;
+  const int *particles;
+  const int *forced_order;
+  const double *forced_uniforms;
+  const int *forced_steps;
+  int *out_samples;
+  cd *trace;
+  int ldu;
 };
 
-__global__ void gccb_set_params_kernel(GccbParams *p, uint64_t seed, double eta) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    p->seed = seed;
-    p->eta = eta;
-  }
+__global__ void gccb_set_params_kernel(GccbParams *p, GccbParams v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 __device__ __forceinline__ double binom_d(int n, int k) {
@@ -90,13 +97,14 @@ __device__ __forceinline__ double binom_d(int n, int k) {
 
 // Per sample: particle order, number of steps, zeroed output row.
 __global__ void gccb_init_kernel(const GccbParams *__restrict__ prm, int m, int n, int S,
-                                 const int *__restrict__ forced_order,
-                                 const int *__restrict__ forced_steps, int *__restrict__ order,
-                                 int *__restrict__ steps, int *__restrict__ out_samples) {
+                                 int *__restrict__ order, int *__restrict__ steps) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= S) return;
   const uint64_t seed = prm->seed;
   const double eta = prm->eta;
+  const int *forced_order = prm->forced_order;
+  const int *forced_steps = prm->forced_steps;
+  int *out_samples = prm->out_samples;
   int *ord = order + (size_t)s * n;
   if (forced_order) {
     for (int i = 0; i < n; ++i) ord[i] = forced_order[(size_t)s * n + i];
@@ -167,12 +175,12 @@ __global__ __launch_bounds__(1024) void gccb_order_kernel(const int *__restrict_
 }
 
 // U^T restricted to input particles: UinT[nu][mode] = U[mode][particles[nu]].
-__global__ void gccb_uin_kernel(const cd *__restrict__ U, int ldu, const int *__restrict__ particles,
-                                int m, int n, cd *__restrict__ UinT) {
+__global__ void gccb_uin_kernel(const GccbParams *__restrict__ prm, int m, int n,
+                                cd *__restrict__ UinT) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= m * n) return;
   const int nu = idx / m, mm = idx - nu * m;
-  UinT[idx] = U[(size_t)mm * ldu + particles[nu]];
+  UinT[idx] = prm->U[(size_t)mm * prm->ldu + prm->particles[nu]];
 }
 
 // One wave per active slot at step k: Laplace over m modes, inverse-CDF draw, output and
@@ -180,10 +188,11 @@ __global__ void gccb_uin_kernel(const cd *__restrict__ U, int ldu, const int *__
 __global__ __launch_bounds__(64) void gccb_step_kernel(
     const GccbParams *__restrict__ prm, int k, int m, int n, int S, const int *__restrict__ count,
     const int *__restrict__ slot_map, const cd *__restrict__ minors, const cd *__restrict__ UinT,
-    const int *__restrict__ order, const double *__restrict__ forced_uniforms,
-    int *__restrict__ outs, cd *__restrict__ Vt, int *__restrict__ out_samples,
-    cd *__restrict__ trace) {
+    const int *__restrict__ order, int *__restrict__ outs, cd *__restrict__ Vt) {
   extern __shared__ char smem[];
+  const double *forced_uniforms = prm->forced_uniforms;
+  int *out_samples = prm->out_samples;
+  cd *trace = prm->trace;
   cd *mi = (cd *)smem;                 // k minors
   double *pm = (double *)(mi + n);     // m probabilities
   const int slot = blockIdx.x;
@@ -301,9 +310,7 @@ Layout layout(const tb_gccb_config &c) {
   return L;
 }
 
-int32_t enqueue(const tb_gccb_config &c, const tb::cd *U, int ldu, const int *particles,
-                const int *forced_order, const double *forced_uniforms, const int *forced_steps,
-                int *out_samples, tb::cd *trace, char *ws, hipStream_t s) {
+int32_t enqueue(const tb_gccb_config &c, char *ws, hipStream_t s) {
   const Layout L = layout(c);
   const int S = c.samples, n = c.n, m = c.m;
   tb::GccbParams *prm = (tb::GccbParams *)(ws + L.params);
@@ -316,10 +323,10 @@ int32_t enqueue(const tb_gccb_config &c, const tb::cd *U, int ldu, const int *pa
   tb::cd *Vt = (tb::cd *)(ws + L.vt);
   tb::cd *minors = (tb::cd *)(ws + L.minors);
   tb::cd *partial = (tb::cd *)(ws + L.partial);
-  hipLaunchKernelGGL(tb::gccb_uin_kernel, dim3((m * n + 255) / 256), dim3(256), 0, s, U, ldu,
-                     particles, m, n, UinT);
+  hipLaunchKernelGGL(tb::gccb_uin_kernel, dim3((m * n + 255) / 256), dim3(256), 0, s, prm, m, n,
+                     UinT);
   hipLaunchKernelGGL(tb::gccb_init_kernel, dim3((S + 255) / 256), dim3(256), 0, s, prm, m, n, S,
-                     forced_order, forced_steps, order, steps, out_samples);
+                     order, steps);
   hipLaunchKernelGGL(tb::gccb_order_kernel, dim3(1), dim3(1024), 0, s, steps, S, n, slot_map,
                      count);
   int32_t rc = tbh::check_hip(hipGetLastError(), "gccb_init");
@@ -332,25 +339,25 @@ int32_t enqueue(const tb_gccb_config &c, const tb::cd *U, int ldu, const int *pa
       if (rc) return rc;
     }
     hipLaunchKernelGGL(tb::gccb_step_kernel, dim3(S), dim3(64), step_lds, s, prm, k, m, n, S, count,
-                       slot_map, minors, UinT, order, forced_uniforms, outs, Vt, out_samples,
-                       trace);
+                       slot_map, minors, UinT, order, outs, Vt);
     rc = tbh::check_hip(hipGetLastError(), "gccb_step_kernel");
     if (rc) return rc;
   }
   return TB_OK;
 }
 
-// Graph cache: one instantiated graph per distinct launch configuration.
+// Graph cache: one instantiated graph per shape (the caller's buffers are read from the
+// parameter block), bounded so that callers with ever-new workspaces cannot grow it
+// without limit.
 struct Key {
-  int device, m, n, S, ldu;
+  int device, m, n, S;
   std::vector<int> chunk;
-  const void *U, *particles, *fo, *fu, *fs, *out, *trace, *ws;
+  const void *ws;
   bool operator<(const Key &o) const {
-    return std::tie(device, m, n, S, ldu, chunk, U, particles, fo, fu, fs, out, trace, ws) <
-           std::tie(o.device, o.m, o.n, o.S, o.ldu, o.chunk, o.U, o.particles, o.fo, o.fu, o.fs,
-                    o.out, o.trace, o.ws);
+    return std::tie(device, m, n, S, chunk, ws) < std::tie(o.device, o.m, o.n, o.S, o.chunk, o.ws);
   }
 };
+constexpr size_t kMaxGraphs = 32;
 
 std::mutex g_mu;
 std::map<Key, hipGraphExec_t> g_graphs;
@@ -402,21 +409,44 @@ extern "C" int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128
   if (step_lds > 64 * 1024) return tbh::fail(TB_ERANGE, "gccb: m=%d too large for LDS pmf", c.m);
   char *ws = (char *)workspace;
   const Layout L = layout(c);
+  tb::GccbParams prm_v{};
+  prm_v.seed = seed;
+  prm_v.eta = transmissivity;
+  prm_v.U = (const tb::cd *)U;
+  prm_v.ldu = ldu;
+  prm_v.particles = particles;
+  prm_v.forced_order = forced_order;
+  prm_v.forced_uniforms = forced_uniforms;
+  prm_v.forced_steps = forced_steps;
+  prm_v.out_samples = out_samples;
+  prm_v.trace = (tb::cd *)trace_minors;
   hipLaunchKernelGGL(tb::gccb_set_params_kernel, dim3(1), dim3(1), 0, s,
-                     (tb::GccbParams *)(ws + L.params), seed, transmissivity);
+                     (tb::GccbParams *)(ws + L.params), prm_v);
   int32_t rc = tbh::check_hip(hipGetLastError(), "gccb_set_params_kernel");
   if (rc) return rc;
 
   int dev = 0;
   rc = tbh::check_hip(hipGetDevice(&dev), "hipGetDevice");
   if (rc) return rc;
-  Key key{dev, c.m, c.n, c.samples, ldu,
-          std::vector<int>(c.chunk_bits, c.chunk_bits + c.n + 1),
-          U, particles, forced_order, forced_uniforms, forced_steps, out_samples, trace_minors,
+  Key key{dev, c.m, c.n, c.samples, std::vector<int>(c.chunk_bits, c.chunk_bits + c.n + 1),
           workspace};
   std::lock_guard<std::mutex> lock(g_mu);
   auto it = g_graphs.find(key);
   if (it == g_graphs.end()) {
+    size_t on_dev = 0;
+    for (auto &kv : g_graphs) on_dev += (kv.first.device == dev);
+    if (on_dev >= kMaxGraphs) {  // drop this device's graphs once none can be in flight
+      rc = tbh::check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+      if (rc) return rc;
+      for (auto g = g_graphs.begin(); g != g_graphs.end();) {
+        if (g->first.device == dev) {
+          (void)hipGraphExecDestroy(g->second);
+          g = g_graphs.erase(g);
+        } else {
+          ++g;
+        }
+      }
+    }
     hipStream_t cap;
     auto cs = g_capture_streams.find(dev);
     if (cs == g_capture_streams.end()) {
@@ -429,8 +459,7 @@ extern "C" int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128
     rc = tbh::check_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal),
                         "hipStreamBeginCapture");
     if (rc) return rc;
-    int32_t erc = enqueue(c, (const tb::cd *)U, ldu, particles, forced_order, forced_uniforms,
-                          forced_steps, out_samples, (tb::cd *)trace_minors, ws, cap);
+    int32_t erc = enqueue(c, ws, cap);
     hipGraph_t graph = nullptr;
     rc = tbh::check_hip(hipStreamEndCapture(cap, &graph), "hipStreamEndCapture");
     if (erc) {

@@ -113,3 +113,104 @@ class BSDistributionCalculatorWithUniformLosses(BSDistributionCalculatorWithFixe
             for j, i in enumerate(nonempty):
                 probs[i] = fixed[j] * self.weights[sum(outcomes[i])]
         return list(probs)
+
+
+class BSSampleBasedDistributionCalculator:
+    """Outcome frequencies of a sampling strategy
+    (bs_sample_based_distribution_calculator.py:18-113).
+
+    The reference scans its outcome list for every sample (:92-106). Here the samples go
+    to the device histogram (tb_outcome_histogram), which ranks each sample directly in the
+    generate_possible_states order; a user-supplied outcome list is mapped onto those
+    ranks, and, as in the reference, only the first occurrence of a repeated outcome
+    collects its count. Strategies with a device sampler (sample_tensor) keep the samples
+    on the GPU; others are simulated on the host and uploaded.
+
+    As in the reference, calculate_distribution() and calculate_probabilities_of_outcomes()
+    draw calculate_approximate_distribution's default of 5000 samples, whatever
+    samples_number holds (:68-75, 77).
+    """
+
+    def __init__(self, experiment_configuration: BosonSamplingExperimentConfiguration,
+                 strategy, samples_number: int = 5000, outcomes=None) -> None:
+        self._configuration = experiment_configuration
+        self._strategy = strategy
+        self._outcomes = outcomes
+        self._samples_number = samples_number
+
+    @property
+    def configuration(self) -> BosonSamplingExperimentConfiguration:
+        return self._configuration
+
+    @configuration.setter
+    def configuration(self, new_configuration: BosonSamplingExperimentConfiguration) -> None:
+        self._configuration = new_configuration
+
+    @property
+    def strategy(self):
+        return self._strategy
+
+    @strategy.setter
+    def strategy(self, new_strategy) -> None:
+        self._strategy = new_strategy
+
+    @property
+    def outcomes(self):
+        return self._outcomes
+
+    @outcomes.setter
+    def outcomes(self, new_outcomes) -> None:
+        self._outcomes = new_outcomes
+
+    @property
+    def samples_number(self) -> int:
+        return self._samples_number
+
+    @samples_number.setter
+    def samples_number(self, new_samples_number: int) -> None:
+        self._samples_number = new_samples_number
+
+    def calculate_distribution(self) -> List[float]:
+        return self.calculate_approximate_distribution()
+
+    def calculate_probabilities_of_outcomes(self, outcomes: Iterable[Iterable[int]]) -> List[float]:
+        self._outcomes = outcomes
+        return self.calculate_approximate_distribution()
+
+    def get_outcomes_in_proper_order(self):
+        return self._outcomes
+
+    def _sample_tensor(self, samples_number: int):
+        from . import _device as D
+
+        T = D.torch()
+        state = self._configuration.initial_state
+        if hasattr(self._strategy, "sample_tensor"):
+            out, _ = self._strategy.sample_tensor(state, samples_number)
+            return out
+        samples = self._strategy.simulate(state, samples_number)
+        host = np.asarray([as_occupation(s) for s in samples], dtype=np.int32).reshape(
+            len(samples), -1)
+        return T.from_numpy(host).to(D.device())
+
+    def calculate_approximate_distribution(self, samples_number: int = 5000) -> List[float]:
+        from . import ops
+        from .states import outcome_rank
+
+        cfg = self._configuration
+        samples = self._sample_tensor(samples_number)
+        if self._outcomes is None:
+            counts, _ = ops.outcome_histogram(samples, cfg.number_of_particles_left)
+            return list((counts.cpu().numpy() / samples_number).astype(np.float64))
+        outcomes = [tuple(int(x) for x in as_occupation(o)) for o in self._outcomes]
+        top = max([sum(o) for o in outcomes] + [0])
+        counts, _ = ops.outcome_histogram(samples, top, losses=True)
+        counts = counts.cpu().numpy()
+        probs = np.zeros(len(outcomes), dtype=np.float64)
+        seen = set()
+        for j, o in enumerate(outcomes):
+            r = outcome_rank(o, top, losses=True)
+            if r >= 0 and o not in seen and len(o) == samples.shape[1]:
+                probs[j] = counts[r]
+            seen.add(o)
+        return list(probs / samples_number)

@@ -139,6 +139,9 @@ def gccb_sample(U, particles, samples: int, seed: int, transmissivity: float = 1
     ws = D.workspace(need, U.device)
     if out is None:
         out = T.empty((samples, m), dtype=T.int32, device=U.device)
+    elif (out.dtype != T.int32 or not out.is_cuda or tuple(out.shape) != (samples, m)
+          or not out.is_contiguous()):
+        raise ValueError(f"out must be a contiguous int32 GPU tensor of shape ({samples}, {m})")
     tr = T.zeros((samples, n, n), dtype=T.complex128, device=U.device) if trace else None
     if forced_order is not None:
         fo_host = forced_order.detach().cpu().numpy()
@@ -168,3 +171,28 @@ def gccb_sample(U, particles, samples: int, seed: int, transmissivity: float = 1
         D.ptr(fs) if fs is not None else None, D.ptr(out), D.ptr(tr) if tr is not None else None,
         D.ptr(ws), need, D.stream_ptr(U.device)))
     return out, tr
+
+
+def outcome_count(particles_number: int, modes_number: int, losses: bool = False) -> int:
+    """Length of generate_possible_states(particles_number, modes_number, losses)."""
+    c = L.load().tb_outcome_count(int(particles_number), int(modes_number), 1 if losses else 0)
+    L.check(min(c, 0))
+    return int(c)
+
+
+def outcome_histogram(samples, particles_number: int, losses: bool = False):
+    """Counts of the samples (S x m int32 GPU tensor) over the outcome list
+    generate_possible_states(particles_number, m, losses), in that order, on the device.
+    Returns (counts: int64 tensor, outside: number of samples not in the list)."""
+    T = D.require_gpu()
+    samples = _i32(samples, "samples")
+    if samples.dim() != 2:
+        raise ValueError(f"samples must be S x m, got {tuple(samples.shape)}")
+    S, m = samples.shape
+    count = outcome_count(particles_number, m, losses)
+    counts = T.zeros(count, dtype=T.int32, device=samples.device)  # uint32 counters
+    outside = T.zeros(1, dtype=T.int32, device=samples.device)
+    L.check(L.load().tb_outcome_histogram(D.ptr(samples), m, S, m, int(particles_number),
+                                          1 if losses else 0, D.ptr(counts), D.ptr(outside),
+                                          D.stream_ptr(samples.device)))
+    return counts.to(T.int64), int(outside.item())

@@ -6,6 +6,7 @@ scattering matrix gather (:545-626) and the 2m loss expansion (:287-342).
 """
 from typing import List, Sequence, Tuple
 
+import math
 import numpy as np
 
 
@@ -114,6 +115,25 @@ def generate_possible_states(particles_number: int, modes_number: int,
         rec([], n, modes_number)
         states.extend(sorted(block, reverse=True))
     return states
+
+
+def outcome_rank(state: Sequence[int], particles_number: int, losses: bool = False) -> int:
+    """Index of `state` in generate_possible_states(particles_number, m, losses), or -1 if
+    it is not in that list. The same rank as the device histogram: within the block of l
+    particles (reverse lexicographic), the states before s are those that agree on modes
+    < i and hold more particles in mode i: C(r - s_i - 1 + m - i - 1, m - i - 1) of them
+    for r particles left; block l starts at C(l - 1 + m, m) when losses are listed."""
+    s = [int(x) for x in state]
+    m, l = len(s), sum(s)
+    if m < 1 or any(x < 0 for x in s) or l > particles_number or (not losses and l != particles_number):
+        return -1
+    rank = math.comb(l - 1 + m, m) if (losses and l > 0) else 0
+    rem = l
+    for i in range(m - 1):
+        if rem - s[i] - 1 >= 0:
+            rank += math.comb(rem - s[i] - 1 + m - i - 1, m - i - 1)
+        rem -= s[i]
+    return rank
 
 
 def generate_lossy_n_particle_input_states(initial_state: Sequence[int],
