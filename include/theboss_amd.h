@@ -197,6 +197,11 @@ int32_t tb_perm_grad_batch(const tb_complex128 *A, int32_t n, int32_t batch,
 typedef struct {
   int32_t m, n, samples;
   int32_t chunk_bits[TB_SWEEP_MAX_K + 1]; /* per step k: Gray bits per lane chunk (0 = auto) */
+  int32_t per_sample; /* 1: sample s has its own interferometer U + s*u_stride and particle
+                         list particles + s*n, whose first forced_steps[s] entries are its
+                         particles (forced_steps required; the rest is padding with valid
+                         mode indices). Used by the BOBS approximation strategies. */
+  int64_t u_stride;   /* elements between per-sample matrices (>= m*ldu) */
 } tb_gccb_config;
 
 size_t tb_gccb_workspace_size(const tb_gccb_config *cfg);

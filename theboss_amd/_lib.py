@@ -48,6 +48,8 @@ class GccbConfig(ctypes.Structure):
         ("n", ctypes.c_int32),
         ("samples", ctypes.c_int32),
         ("chunk_bits", ctypes.c_int32 * (TB_SWEEP_MAX_K + 1)),
+        ("per_sample", ctypes.c_int32),
+        ("u_stride", ctypes.c_int64),
     ]
 
 

@@ -80,7 +80,9 @@ struct GccbParams {
   const int *forced_steps;
   int *out_samples;
   cd *trace;
+  int64_t u_stride;  // per-sample interferometers: sample s uses U + s * u_stride
   int ldu;
+  int per_sample;    // 1: per-sample U and particle lists (particles + s * n)
 };
 
 __global__ void gccb_set_params_kernel(GccbParams *p, GccbParams v) {
@@ -106,11 +108,14 @@ __global__ void gccb_init_kernel(const GccbParams *__restrict__ prm, int m, int 
   const int *forced_steps = prm->forced_steps;
   int *out_samples = prm->out_samples;
   int *ord = order + (size_t)s * n;
+  // Per-sample particle lists hold the sample's forced_steps[s] particles first: only that
+  // prefix is shuffled, so the padding beyond it never enters a step.
+  const int span = (prm->per_sample && forced_steps) ? min(max(forced_steps[s], 0), n) : n;
   if (forced_order) {
     for (int i = 0; i < n; ++i) ord[i] = forced_order[(size_t)s * n + i];
   } else {
     for (int i = 0; i < n; ++i) ord[i] = i;
-    for (int i = n - 1; i >= 1; --i) {  // Fisher-Yates
+    for (int i = span - 1; i >= 1; --i) {  // Fisher-Yates
       const double u = philox_uniform(seed, (uint32_t)s, (uint32_t)i, kPurposeOrder);
       int j = (int)(u * (double)(i + 1));
       j = j > i ? i : j;
@@ -174,13 +179,17 @@ __global__ __launch_bounds__(1024) void gccb_order_kernel(const int *__restrict_
   }
 }
 
-// U^T restricted to input particles: UinT[nu][mode] = U[mode][particles[nu]].
-__global__ void gccb_uin_kernel(const GccbParams *__restrict__ prm, int m, int n,
+// U^T restricted to input particles: UinT[nu][mode] = U[mode][particles[nu]], one copy
+// per sample when the samples have their own interferometers (items = S, else 1).
+__global__ void gccb_uin_kernel(const GccbParams *__restrict__ prm, int m, int n, int items,
                                 cd *__restrict__ UinT) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= m * n) return;
-  const int nu = idx / m, mm = idx - nu * m;
-  UinT[idx] = prm->U[(size_t)mm * prm->ldu + prm->particles[nu]];
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)items * m * n) return;
+  const int s = (int)(idx / ((int64_t)m * n));
+  const int rem = (int)(idx - (int64_t)s * m * n);
+  const int nu = rem / m, mm = rem - nu * m;
+  const cd *U = prm->U + (size_t)s * prm->u_stride;
+  UinT[idx] = U[(size_t)mm * prm->ldu + prm->particles[(size_t)s * n + nu]];
 }
 
 // One wave per active slot at step k: Laplace over m modes, inverse-CDF draw, output and
@@ -200,6 +209,7 @@ __global__ __launch_bounds__(64) void gccb_step_kernel(
   if (slot >= count[k]) return;
   const int s = slot_map[slot];
   const int *ord = order + (size_t)s * n;
+  if (prm->per_sample) UinT += (size_t)s * n * m;
   if (lane < k) mi[lane] = (k == 1) ? cd{1.0, 0.0} : minors[(size_t)slot * k + lane];
   __syncthreads();
   if (trace && lane < k) trace[((size_t)s * n + (k - 1)) * n + lane] = mi[lane];
@@ -295,7 +305,7 @@ Layout layout(const tb_gccb_config &c) {
     return at;
   };
   L.params = take(sizeof(tb::GccbParams));
-  L.uint = take(n * m * sizeof(tb::cd));
+  L.uint = take((c.per_sample ? S : 1) * n * m * sizeof(tb::cd));
   L.order = take(S * n * sizeof(int));
   L.steps = take(S * sizeof(int));
   L.slot_map = take(S * sizeof(int));
@@ -323,8 +333,10 @@ int32_t enqueue(const tb_gccb_config &c, char *ws, hipStream_t s) {
   tb::cd *Vt = (tb::cd *)(ws + L.vt);
   tb::cd *minors = (tb::cd *)(ws + L.minors);
   tb::cd *partial = (tb::cd *)(ws + L.partial);
-  hipLaunchKernelGGL(tb::gccb_uin_kernel, dim3((m * n + 255) / 256), dim3(256), 0, s, prm, m, n,
-                     UinT);
+  const int uin_items = c.per_sample ? S : 1;
+  hipLaunchKernelGGL(tb::gccb_uin_kernel,
+                     dim3((unsigned)(((int64_t)uin_items * m * n + 255) / 256)), dim3(256), 0, s,
+                     prm, m, n, uin_items, UinT);
   hipLaunchKernelGGL(tb::gccb_init_kernel, dim3((S + 255) / 256), dim3(256), 0, s, prm, m, n, S,
                      order, steps);
   hipLaunchKernelGGL(tb::gccb_order_kernel, dim3(1), dim3(1024), 0, s, steps, S, n, slot_map,
@@ -350,11 +362,12 @@ int32_t enqueue(const tb_gccb_config &c, char *ws, hipStream_t s) {
 // parameter block), bounded so that callers with ever-new workspaces cannot grow it
 // without limit.
 struct Key {
-  int device, m, n, S;
+  int device, m, n, S, per_sample;
   std::vector<int> chunk;
   const void *ws;
   bool operator<(const Key &o) const {
-    return std::tie(device, m, n, S, chunk, ws) < std::tie(o.device, o.m, o.n, o.S, o.chunk, o.ws);
+    return std::tie(device, m, n, S, per_sample, chunk, ws) <
+           std::tie(o.device, o.m, o.n, o.S, o.per_sample, o.chunk, o.ws);
   }
 };
 constexpr size_t kMaxGraphs = 32;
@@ -399,6 +412,12 @@ extern "C" int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128
       return tbh::fail(TB_EINVAL, "gccb: chunk_bits[%d]=%d outside 0..16", k, c.chunk_bits[k]);
   if (!U || !particles || !out_samples) return tbh::fail(TB_EINVAL, "gccb: NULL pointer");
   if (ldu < c.m) return tbh::fail(TB_EINVAL, "gccb: ldu=%d < m=%d", ldu, c.m);
+  if (c.per_sample != 0 && c.per_sample != 1)
+    return tbh::fail(TB_EINVAL, "gccb: per_sample=%d must be 0 or 1", c.per_sample);
+  if (c.per_sample && !forced_steps)
+    return tbh::fail(TB_EINVAL, "gccb: per-sample particle lists need forced_steps");
+  if (c.per_sample && c.u_stride < (int64_t)c.m * ldu)
+    return tbh::fail(TB_EINVAL, "gccb: u_stride=%lld < m*ldu", (long long)c.u_stride);
   if (!(transmissivity >= 0.0 && transmissivity <= 1.0))
     return tbh::fail(TB_EINVAL, "gccb: transmissivity %g outside [0,1]", transmissivity);
   const size_t need = layout(c).total;
@@ -420,6 +439,8 @@ extern "C" int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128
   prm_v.forced_steps = forced_steps;
   prm_v.out_samples = out_samples;
   prm_v.trace = (tb::cd *)trace_minors;
+  prm_v.per_sample = c.per_sample ? 1 : 0;
+  prm_v.u_stride = c.per_sample ? c.u_stride : 0;
   hipLaunchKernelGGL(tb::gccb_set_params_kernel, dim3(1), dim3(1), 0, s,
                      (tb::GccbParams *)(ws + L.params), prm_v);
   int32_t rc = tbh::check_hip(hipGetLastError(), "gccb_set_params_kernel");
@@ -428,8 +449,8 @@ extern "C" int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128
   int dev = 0;
   rc = tbh::check_hip(hipGetDevice(&dev), "hipGetDevice");
   if (rc) return rc;
-  Key key{dev, c.m, c.n, c.samples, std::vector<int>(c.chunk_bits, c.chunk_bits + c.n + 1),
-          workspace};
+  Key key{dev, c.m, c.n, c.samples, c.per_sample ? 1 : 0,
+          std::vector<int>(c.chunk_bits, c.chunk_bits + c.n + 1), workspace};
   std::lock_guard<std::mutex> lock(g_mu);
   auto it = g_graphs.find(key);
   if (it == g_graphs.end()) {

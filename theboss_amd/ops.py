@@ -128,12 +128,31 @@ def gccb_config(m: int, n: int, samples: int, chunk_bits: Optional[Sequence[int]
 def gccb_sample(U, particles, samples: int, seed: int, transmissivity: float = 1.0,
                 forced_order=None, forced_uniforms=None, forced_steps=None,
                 chunk_bits: Optional[Sequence[int]] = None, trace: bool = False, out=None):
-    """Batched GCC-B sampling on the device. Returns (samples x m int32, trace or None)."""
+    """Batched GCC-B sampling on the device. Returns (samples x m int32, trace or None).
+
+    U is m x m (all samples) or samples x m x m (one interferometer per sample); in the
+    second form particles is samples x n, row s holding sample s's forced_steps[s]
+    particles first (the rest is padding with valid mode indices), and forced_steps is
+    required."""
     T = D.require_gpu()
     U = _c128(U, "U")
     particles = _i32(particles, "particles")
-    m, n = U.shape[0], particles.shape[0]
+    per_sample = U.dim() == 3
+    if per_sample:
+        if U.shape[0] != samples or U.shape[1] != U.shape[2]:
+            raise ValueError(f"per-sample U must be {samples} x m x m, got {tuple(U.shape)}")
+        if particles.dim() != 2 or particles.shape[0] != samples:
+            raise ValueError("per-sample particles must be samples x n")
+        if forced_steps is None:
+            raise ValueError("per-sample particle lists need forced_steps")
+        m, n = U.shape[1], particles.shape[1]
+    else:
+        if U.dim() != 2 or U.shape[0] != U.shape[1] or particles.dim() != 1:
+            raise ValueError("U must be m x m and particles a vector of mode indices")
+        m, n = U.shape[0], particles.shape[0]
     cfg = gccb_config(m, n, samples, chunk_bits)
+    cfg.per_sample = 1 if per_sample else 0
+    cfg.u_stride = m * m if per_sample else 0
     lib = L.load()
     need = lib.tb_gccb_workspace_size(ctypes.byref(cfg))
     ws = D.workspace(need, U.device)
@@ -148,6 +167,12 @@ def gccb_sample(U, particles, samples: int, seed: int, transmissivity: float = 1
         if fo_host.shape != (samples, n) or not all(
                 np.array_equal(np.sort(row), np.arange(n)) for row in fo_host):
             raise ValueError("forced_order must hold one permutation of range(n) per sample")
+        if per_sample:
+            fs_chk = forced_steps.detach().cpu().numpy()
+            if not all(np.array_equal(np.sort(row[:l]), np.arange(l))
+                       for row, l in zip(fo_host, fs_chk)):
+                raise ValueError("with per-sample particles, forced_order row s must start "
+                                 "with a permutation of range(forced_steps[s])")
     if forced_uniforms is not None:
         fu_host = forced_uniforms.detach().cpu().numpy()
         if fu_host.shape != (samples, n) or np.any(fu_host < 0) or np.any(fu_host >= 1):
@@ -165,7 +190,7 @@ def gccb_sample(U, particles, samples: int, seed: int, transmissivity: float = 1
         raise TypeError("forced_uniforms must be a float64 GPU tensor")
     fs = _i32(forced_steps, "forced_steps") if forced_steps is not None else None
     L.check(lib.tb_gccb_sample(
-        ctypes.byref(cfg), D.ptr(U), U.shape[1], D.ptr(particles), ctypes.c_uint64(seed & (2**64 - 1)),
+        ctypes.byref(cfg), D.ptr(U), U.shape[-1], D.ptr(particles), ctypes.c_uint64(seed & (2**64 - 1)),
         float(transmissivity),
         D.ptr(fo) if fo is not None else None, D.ptr(fu) if fu is not None else None,
         D.ptr(fs) if fs is not None else None, D.ptr(out), D.ptr(tr) if tr is not None else None,

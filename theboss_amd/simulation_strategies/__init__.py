@@ -1,5 +1,9 @@
 """Sampling strategies built on the GPU permanent calculators."""
 from .batched_gccb_simulation_strategy import BatchedGCCBSimulationStrategy
+from .bobs_strategies import (
+    LossyStateApproximationSimulationStrategy,
+    NonuniformLossesApproximationStrategy,
+)
 from .generalized_cliffords_b_simulation_strategy import GeneralizedCliffordsBSimulationStrategy
 from .generalized_cliffords_simulation_strategy import GeneralizedCliffordsSimulationStrategy
 from .lossy_strategies import (
