@@ -14,7 +14,8 @@ per-sample mode (tb_gccb_sample with per_sample = 1) draws every sample.
   lossy interferometer by SVD, lossy-network GCC-B on the 2m-mode expansion).
 
 The host randomness (loss draws, column permutations, phases, symmetrisation) comes from a
-seeded numpy Generator; the GCC-B draws from the device Philox stream.
+seeded numpy Generator; the per-sample matrices are formed on the device (batched complex
+matrix products); the GCC-B draws come from the device Philox stream.
 """
 import math
 from typing import List, Sequence, Tuple
@@ -38,13 +39,15 @@ def generate_qft_matrix_for_first_m_modes(m: int, all_modes_number: int) -> np.n
 
 def _particle_lists(inputs: np.ndarray, n_max: int) -> Tuple[np.ndarray, np.ndarray]:
     """Mode assignment of each input row (sorted modes with multiplicity), padded with mode
-    0 to n_max entries, and the particle counts."""
+    0 to n_max entries, and the particle counts: particle p of a row sits in the first
+    mode whose running occupation exceeds p."""
     counts = inputs.sum(axis=1).astype(np.int32)
-    lists = np.zeros((inputs.shape[0], max(n_max, 1)), dtype=np.int32)
-    for s, row in enumerate(inputs):
-        modes = np.repeat(np.arange(len(row)), row)
-        lists[s, : len(modes)] = modes
-    return lists, counts
+    width = max(n_max, 1)
+    cum = np.cumsum(inputs, axis=1)                                   # S x m
+    p = np.arange(width)
+    lists = (cum[:, :, None] <= p[None, None, :]).sum(axis=1)         # S x width
+    lists = np.where(p[None, :] < counts[:, None], lists, 0)
+    return lists.astype(np.int32), counts
 
 
 def expand_batch(Us: np.ndarray) -> np.ndarray:
@@ -70,18 +73,27 @@ def expand_batch(Us: np.ndarray) -> np.ndarray:
     return ev @ es @ ew
 
 
-def _sample_on_device(Us: np.ndarray, inputs: np.ndarray, seed: int) -> np.ndarray:
-    """One device call: sample s uses interferometer Us[s] and input occupation inputs[s]."""
+def _sample_on_device(Us, inputs: np.ndarray, seed: int) -> np.ndarray:
+    """One device call: sample s uses interferometer Us[s] (S x M x M complex128 device
+    tensor) and input occupation inputs[s]."""
     T = D.torch()
     S = Us.shape[0]
     n_max = int(inputs.sum(axis=1).max()) if S else 0
     lists, counts = _particle_lists(inputs, n_max)
-    dev = D.device()
-    U_d = D.to_device_c128(np.ascontiguousarray(Us), dev)
+    dev = Us.device
     p_d = D.to_device_i32(lists, dev)
     steps = T.from_numpy(counts).to(dev)
-    out, _ = ops.gccb_sample(U_d, p_d, S, seed, forced_steps=steps)
+    out, _ = ops.gccb_sample(Us.contiguous(), p_d, S, seed, forced_steps=steps)
     return out.cpu().numpy()
+
+
+def _phases(rng, S: int, m: int, k: int) -> np.ndarray:
+    """exp(2 pi i u) on the first k modes, 1 elsewhere
+    (generate_random_phases_matrix_for_first_m_modes, boson_sampling_utilities.py:524-542)."""
+    ph = np.ones((S, m), dtype=np.complex128)
+    if k > 0:
+        ph[:, :k] = np.exp(2j * np.pi * rng.random((S, k)))
+    return ph
 
 
 class LossyStateApproximationSimulationStrategy(SimulationStrategyInterface):
@@ -146,17 +158,21 @@ class LossyStateApproximationSimulationStrategy(SimulationStrategyInterface):
         S = samples_number
         pi = self._choose(weights, S)
         ai = self._choose(aweights, S)
-        inputs = np.array([list(parts[i]) + list(aparts[j]) for i, j in zip(pi, ai)],
-                          dtype=np.int64).reshape(S, -1)
+        P = np.array(parts, dtype=np.int64).reshape(len(parts), -1)
+        A = np.array(aparts, dtype=np.int64).reshape(len(aparts), -1)
+        inputs = np.concatenate([P[pi], A[ai]], axis=1)
+        T = D.torch()
         U = np.asarray(self._permanent_calculator.matrix, dtype=np.complex128)
         m = U.shape[0]
         k = m - self._hierarchy_level  # modes under phases and QFT
         perms = np.argsort(self._rng.random((S, m)), axis=1)
-        Us = U[:, perms].transpose(1, 0, 2)  # Us[s] = U[:, perms[s]]
-        phases = np.ones((S, m), dtype=np.complex128)
-        if k > 0:
-            phases[:, :k] = np.exp(2j * np.pi * self._rng.random((S, k)))
-        Us = (Us * phases[:, None, :]) @ self._qft_matrix
+        phases = _phases(self._rng, S, m, k)
+        # On the device: U[:, perm_s] diag(phases_s) QFT for every sample.
+        dev = D.device()
+        U_d = D.to_device_c128(U, dev)
+        cols = U_d[:, T.from_numpy(perms.reshape(-1)).to(dev)].reshape(m, S, m).permute(1, 0, 2)
+        scaled = (cols * D.to_device_c128(phases, dev)[:, None, :]).contiguous()
+        Us = T.matmul(scaled, D.to_device_c128(self._qft_matrix, dev))
         seed = (self._seed * 0x9E3779B97F4A7C15 + self._calls) & (2 ** 64 - 1)
         self._calls += 1
         out = _sample_on_device(Us, inputs, seed)
@@ -194,6 +210,12 @@ class NonuniformLossesApproximationStrategy(SimulationStrategyInterface):
         self._uniform_losses = float(np.min(1 - transmissivities))
         eta = transmissivities / (1 - self._uniform_losses)
         self._initial_matrix = u @ np.diag(np.sqrt(eta)) @ v
+        # The SVD of initial_matrix @ P is (u, sqrt(eta), v @ P) for any unitary P, so the
+        # per-sample dilations need no per-sample SVD: their first m columns are
+        # [u diag(sqrt(eta)); diag(sqrt(1 - eta))] @ v @ P.
+        self._dilation_left = np.concatenate(
+            [u * np.sqrt(eta)[None, :], np.diag(np.sqrt(np.clip(1.0 - eta, 0.0, None)))], axis=0)
+        self._right = v
 
     def _binomial_weights(self, max_particles: int):
         eta = 1 - self._uniform_losses
@@ -213,23 +235,28 @@ class NonuniformLossesApproximationStrategy(SimulationStrategyInterface):
         if k > 0:  # symmetrisation: all approximated particles into one random mode
             inputs[np.arange(S), self._rng.integers(0, k, S)] = approx_n
         if not np.isclose(self._uniform_losses, 0):
+            # per mode holding c particles: the first l whose cumulative binomial weight
+            # exceeds a uniform draw (the reference's choice(), :31-62)
             weights = self._binomial_weights(int(max(state.max(), approx_n, 1)))
             u = self._rng.random((S, m))
             lossy = np.zeros_like(inputs)
-            for mode in range(m):
-                for s in range(S):
-                    c = int(inputs[s, mode])
-                    if c == 0:
-                        continue
-                    cdf = np.cumsum(weights[c])
-                    lossy[s, mode] = min(int(np.searchsorted(cdf, u[s, mode], side="right")), c)
+            for c in np.unique(inputs[inputs > 0]):
+                cdf = np.cumsum(weights[int(c)])
+                hit = inputs == c
+                lossy[hit] = np.minimum(np.searchsorted(cdf, u[hit], side="right"), c)
             inputs = lossy
         qft = generate_qft_matrix_for_first_m_modes(k, m)
-        phases = np.ones((S, m), dtype=np.complex128)
-        if k > 0:
-            phases[:, :k] = np.exp(2j * np.pi * self._rng.random((S, k)))
-        Us = (self._initial_matrix[None, :, :] * phases[:, None, :]) @ qft
-        expanded = expand_batch(Us)
+        phases = _phases(self._rng, S, m, k)
+        # On the device: the 2m-mode dilation of initial_matrix diag(phases_s) QFT. Only its
+        # first m columns (the input modes) are ever read; the rest stay zero.
+        T = D.torch()
+        dev = D.device()
+        right = (D.to_device_c128(self._right, dev)[None, :, :]
+                 * D.to_device_c128(phases, dev)[:, None, :]).contiguous()
+        right = T.matmul(right, D.to_device_c128(qft, dev))                    # S x m x m
+        left = D.to_device_c128(self._dilation_left, dev)                      # 2m x m
+        expanded = T.zeros((S, 2 * m, 2 * m), dtype=T.complex128, device=dev)
+        expanded[:, :, :m] = T.matmul(left.expand(S, 2 * m, m).contiguous(), right)
         seed = (self._seed * 0x9E3779B97F4A7C15 + self._calls) & (2 ** 64 - 1)
         self._calls += 1
         padded = np.concatenate([inputs, np.zeros_like(inputs)], axis=1)

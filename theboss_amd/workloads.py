@@ -207,3 +207,37 @@ def hist(checker):
             "frac_hbm_peak": algo / t / 8e12,
             "counted": int(counts.sum().item()), "outside": outside,
             "distinct_outcomes_hit": int((counts > 0).sum().item())}
+
+
+def bobs(checker):
+    """BOBS non-uniform-loss approximation: m=16 Haar modes with transmissivity 0.5..0.95
+    by output mode, 8 photons in modes 0..7, 4 approximated modes, 4096 samples in one
+    device call; against one sample per call (the reference's per-sample loop)."""
+    from .permanent_calculators import GPURyserPermanentCalculator
+    from .simulation_strategies import NonuniformLossesApproximationStrategy
+
+    T = D.torch()
+    m, n, S, k = 16, 8, 4096, 4
+    U = checker.haar_unitary(m, 31)
+    eta = np.linspace(0.5, 0.95, m)
+    lossy = np.sqrt(eta)[:, None] * U
+    init = [1] * n + [0] * (m - n)
+    strat = NonuniformLossesApproximationStrategy(GPURyserPermanentCalculator(lossy), k, seed=3)
+    strat.simulate(init, S)  # capture + warm
+    T.cuda.synchronize()
+    t0 = time.perf_counter()
+    samples = strat.simulate(init, S)
+    T.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    one = NonuniformLossesApproximationStrategy(GPURyserPermanentCalculator(lossy), k, seed=4)
+    one.simulate(init, 1)
+    T.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(16):
+        one.simulate(init, 1)
+    T.cuda.synchronize()
+    t_one = (time.perf_counter() - t0) / 16
+    return {"workload": "BOBS non-uniform losses, m=16, n=8, 4 approximated modes, 4096 samples",
+            "samples_per_s": S / t_batch, "ms_per_batch": t_batch * 1e3,
+            "one_per_call_samples_per_s": 1.0 / t_one, "batched_vs_one_per_call": t_one * S / t_batch,
+            "mean_photons": float(np.mean([sum(s_) for s_ in samples]))}
