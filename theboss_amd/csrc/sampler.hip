@@ -204,32 +204,38 @@ __global__ __launch_bounds__(64) void gccb_step_kernel(
   cd *trace = prm->trace;
   cd *mi = (cd *)smem;                 // k minors
   double *pm = (double *)(mi + n);     // m probabilities
+  int *rows = (int *)(pm + m);         // UinT row of each of the k particles
   const int slot = blockIdx.x;
   const int lane = threadIdx.x;
   if (slot >= count[k]) return;
   const int s = slot_map[slot];
   const int *ord = order + (size_t)s * n;
   if (prm->per_sample) UinT += (size_t)s * n * m;
-  if (lane < k) mi[lane] = (k == 1) ? cd{1.0, 0.0} : minors[(size_t)slot * k + lane];
+  if (lane < k) {
+    mi[lane] = (k == 1) ? cd{1.0, 0.0} : minors[(size_t)slot * k + lane];
+    rows[lane] = ord[lane];
+  }
   __syncthreads();
   if (trace && lane < k) trace[((size_t)s * n + (k - 1)) * n + lane] = mi[lane];
-  // pmf: amp_m = sum_nu U[m][in_nu] minor_nu over the current k particles.
-  const int per = (m + 63) / 64;
-  const int m0 = lane * per;
-  const int m1 = min(m, m0 + per);
-  double local = 0.0;
-  for (int mm = m0; mm < m1; ++mm) {
+  // pmf: amp_m = sum_nu U[m][in_nu] minor_nu over the current k particles. Lanes take
+  // modes lane, lane + 64, ... so each UinT row is read with coalesced loads.
+  for (int mm = lane; mm < m; mm += 64) {
     double re = 0.0, im = 0.0;
     for (int nu = 0; nu < k; ++nu) {
-      const cd u = UinT[(size_t)ord[nu] * m + mm];
+      const cd u = UinT[(size_t)rows[nu] * m + mm];
       const cd v = mi[nu];
       re += u.re * v.re - u.im * v.im;
       im += u.re * v.im + u.im * v.re;
     }
-    const double p = re * re + im * im;
-    pm[mm] = p;
-    local += p;
+    pm[mm] = re * re + im * im;
   }
+  __syncthreads();
+  // Inverse CDF over contiguous mode ranges per lane.
+  const int per = (m + 63) / 64;
+  const int m0 = lane * per;
+  const int m1 = min(m, m0 + per);
+  double local = 0.0;
+  for (int mm = m0; mm < m1; ++mm) local += pm[mm];
   // Inclusive scan of lane sums.
   double incl = local;
 #pragma unroll
@@ -343,7 +349,8 @@ int32_t enqueue(const tb_gccb_config &c, char *ws, hipStream_t s) {
                      count);
   int32_t rc = tbh::check_hip(hipGetLastError(), "gccb_init");
   if (rc) return rc;
-  const size_t step_lds = (size_t)n * sizeof(tb::cd) + (size_t)m * sizeof(double);
+  const size_t step_lds =
+      (size_t)n * sizeof(tb::cd) + (size_t)m * sizeof(double) + (size_t)n * sizeof(int);
   for (int k = 1; k <= n; ++k) {
     if (k >= 2) {
       rc = tbs::run_sweep(k, Vt, n, (int64_t)n * n, slot_map, count + k, S, c.chunk_bits[k],
@@ -424,7 +431,8 @@ extern "C" int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128
   if (!workspace || workspace_bytes < need)
     return tbh::fail(TB_EWORKSPACE, "gccb: workspace %zu bytes < required %zu", workspace_bytes,
                      need);
-  const size_t step_lds = (size_t)c.n * sizeof(tb::cd) + (size_t)c.m * sizeof(double);
+  const size_t step_lds =
+      (size_t)c.n * sizeof(tb::cd) + (size_t)c.m * sizeof(double) + (size_t)c.n * sizeof(int);
   if (step_lds > 64 * 1024) return tbh::fail(TB_ERANGE, "gccb: m=%d too large for LDS pmf", c.m);
   char *ws = (char *)workspace;
   const Layout L = layout(c);
