@@ -143,6 +143,25 @@ def test_guan_random_multiplicities_batch(T, oracle):
         assert np.max(np.abs(got - ref)) <= 1e-12, cls.__name__
 
 
+def test_guan_outer_batch_vs_oracle(T, oracle):
+    """All (output, input) pairs of two lists of 4-particle occupations, with collisions
+    on either side so that both the direct and the transposed launches run."""
+    from theboss_amd.permanent_calculators import GPUChinHuhPermanentCalculator, GPURyserPermanentCalculator
+
+    rng = np.random.default_rng(6)
+    m, k = 6, 4
+    U = oracle.haar_unitary(m, 98)
+    S = np.array([np.bincount(rng.integers(0, m, k), minlength=m) for _ in range(9)] + [[4, 0, 0, 0, 0, 0]])
+    Tm = np.array([np.bincount(rng.integers(0, m, k), minlength=m) for _ in range(7)] + [[1, 1, 1, 1, 0, 0]])
+    ref = np.array([[oracle.perm_classic(U, s, t) for s in S] for t in Tm])
+    for cls in (GPURyserPermanentCalculator, GPUChinHuhPermanentCalculator):
+        got = cls(U).compute_permanent_outer(S, Tm).cpu().numpy()
+        assert got.shape == (len(Tm), len(S))
+        assert np.max(np.abs(got - ref)) <= 1e-12, cls.__name__
+    with pytest.raises(AttributeError):
+        GPURyserPermanentCalculator(U).compute_permanent_outer(S, np.array([[1, 0, 0, 0, 0, 0]]))
+
+
 def test_shape_mismatch_raises_attribute_error(T, oracle):
     from theboss_amd.permanent_calculators import GPURyserPermanentCalculator
 

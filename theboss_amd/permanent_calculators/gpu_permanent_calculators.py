@@ -161,6 +161,41 @@ class _GuanGPUCalculator(BSPermanentCalculatorBase):
             res[idx] = r.cpu().numpy()
         return res
 
+    def compute_permanent_outer(self, S: np.ndarray, Tm: np.ndarray):
+        """All pairs at once: a (len(Tm), len(S)) complex128 device tensor holding
+        perm(U[Tm[j], S[i]]) at [j, i], for S and Tm rows that all hold the same number
+        k >= 1 of particles. Each side is compacted once on the host; the per-pair mode
+        lists are gathered on the device. None when a side needs the Glynn fallback."""
+        S = np.asarray(S, dtype=np.int64)
+        Tm = np.asarray(Tm, dtype=np.int64)
+        k = int(S[0].sum()) if len(S) else 0
+        if k < 1 or np.any(S.sum(axis=1) != k) or np.any(Tm.sum(axis=1) != k):
+            raise AttributeError("outer batch needs rows with one common particle number >= 1")
+        T = D.require_gpu()
+        dev = D.device()
+        ns, nt = S.shape[0], Tm.shape[0]
+        codes_s = np.prod(S.astype(np.float64) + 1, axis=1)
+        codes_t = np.prod(Tm.astype(np.float64) + 1, axis=1)
+        cs = [D.to_device_i32(a, dev) for a in compact_batch(S)]
+        ct = [D.to_device_i32(a, dev) for a in compact_batch(Tm)]
+        if max(cs[0].shape[1], ct[0].shape[1]) > min(L.TB_GUAN_MAX_COLS, L.TB_GUAN_MAX_ROWS) \
+                or max(codes_s.max(), codes_t.max()) > 2.0 ** 28:
+            return None
+        U = np.asarray(self._matrix, dtype=np.complex128)
+        flip = T.from_numpy(codes_t[:, None] < codes_s[None, :]).to(dev).reshape(-1)
+        res = T.empty(nt * ns, dtype=T.complex128, device=dev)
+        for transposed in (False, True):
+            idx = T.nonzero(flip == transposed).reshape(-1)
+            if idx.numel() == 0:
+                continue
+            j, i = idx // ns, idx % ns
+            (em, eu), (pm, pu) = ((ct, cs) if transposed else (cs, ct))
+            ei, pi = (j, i) if transposed else (i, j)
+            M = U.T if transposed else U
+            res[idx] = ops.guan_perm_batch(D.to_device_c128(M, dev), pm[pi], pu[pi], em[ei],
+                                           eu[ei], self.formula)
+        return res.reshape(nt, ns)
+
     def compute_permanent(self) -> complex128:
         s, t = self._checked_states()
         if len(s) == 0:
@@ -179,9 +214,9 @@ class _GuanGPUCalculator(BSPermanentCalculatorBase):
     def compute_permanent_batch_arrays(self, S: np.ndarray, Tm: np.ndarray) -> np.ndarray:
         S = np.asarray(S, dtype=np.int64)
         Tm = np.asarray(Tm, dtype=np.int64)
-        pairs = list(zip(S, Tm))
         res = self._launch(S, Tm)
         if res is None:
+            pairs = list(zip(S, Tm))
             # Too many distinct modes for the mixed-radix kernel: expand multiplicities
             # and use the Glynn kernel on the effective matrices (same permanent).
             glynn = GPUGlynnGrayPermanentCalculator(self._matrix)

@@ -5,9 +5,10 @@ Restates theboss's GeneralizedCliffordsSimulationStrategy
 the output is built one particle at a time; layer k needs, for every candidate output
 r + e_o (o < m) and every k-particle substate s of the input, the probability
 |perm(U[r + e_o, s])|^2 / (prod s! k!) weighted by k!(n-k)!/n! prod C(n_i, k_i) (:136-247).
-Layers are memoised by the r-prefix (:28-30, 132-133, 171). The reference computes each
-permanent with a separate calculator call; here all m x C(n, k) permanents of a layer go
-to the GPU in one batched launch of the calculator's compute_permanent_batch.
+Layers are memoised by the r-prefix (:28-30, 132-133, 171). The reference draws the
+samples one after the other and computes each permanent with a separate calculator call;
+here the samples advance together, and all permanents of the layers a round needs
+(m x C(n, k) per new prefix) go to the GPU in one batched launch.
 """
 from collections import defaultdict
 from math import comb, factorial
@@ -15,6 +16,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 
+from .. import _device as D
 from ..states import as_occupation
 from .simulation_strategy_interface import SimulationStrategyInterface
 
@@ -47,6 +49,9 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
         self._bs_permanent_calculator.matrix = new_matrix
 
     def simulate(self, input_state: Sequence[int], samples_number: int = 1) -> List[Tuple[int, ...]]:
+        """All samples advance together, one particle per round: the layers a round needs
+        (one per distinct r-prefix not yet memoised) go to the GPU in one launch. Each
+        sample then draws from its prefix's pmf as the reference does (:249-266)."""
         self.input_state = [int(x) for x in as_occupation(input_state)]
         self.number_of_input_photons = sum(self.input_state)
         self._labeled_states = defaultdict(list)
@@ -54,11 +59,24 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
             self._labeled_states[sum(s)].append(s)
         self._layer_cache = {}
         self.pmfs = {}
-        samples = []
-        while len(samples) < samples_number:
-            self._fill_r_sample()
-            samples.append(tuple(self.r_sample))
-        return samples
+        m, S = len(self.input_state), max(int(samples_number), 0)
+        current = np.zeros((S, m), dtype=np.int64)
+        keys = [tuple([0] * m)] * S
+        for _ in range(self.number_of_input_photons if S else 0):
+            missing = sorted({k for k in keys if k not in self.pmfs})
+            if missing:
+                self._calculate_layers(missing)
+            # the reference's draw, all samples at once: first index whose running sum
+            # (sequential, as cumsum) exceeds random() * sum(pmf)
+            cum = np.cumsum(np.stack([self.pmfs[k] for k in keys]), axis=1)
+            value = self._rng.random(S) * cum[:, -1]
+            idx = np.minimum((cum <= value[:, None]).sum(axis=1), m - 1)
+            current[np.arange(S), idx] += 1
+            keys = list(map(tuple, current.tolist()))
+        if S:
+            self.r_sample = list(keys[-1])
+            self.current_key = keys[-1]
+        return keys
 
     def _fill_r_sample(self) -> None:
         self.r_sample = [0 for _ in self.input_state]
@@ -95,17 +113,34 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
         return self._layer_cache[k]
 
     def _calculate_new_layer_of_pmfs(self) -> None:
-        k = sum(self.r_sample) + 1
+        self._calculate_layers([self.current_key])
+
+    def _calculate_layers(self, keys: List[Tuple[int, ...]]) -> None:
+        """pmfs of the given r-prefixes (all with the same particle count) from one batched
+        launch of m * |substates| permanents per prefix (:136-171)."""
+        k = sum(keys[0]) + 1
         S, weights, norm = self._layer_constants(k)
-        m = len(self.r_sample)
-        outputs = np.tile(np.asarray(self.r_sample, dtype=np.int64), (m, 1)) + np.eye(m, dtype=np.int64)
-        self.possible_outputs[self.current_key] = [list(map(int, o)) for o in outputs]
-        ins = np.tile(S, (m, 1))                      # output-major: item = o * |S| + i
-        outs = np.repeat(outputs, S.shape[0], axis=0)
-        perms = self._bs_permanent_calculator.compute_permanent_batch_arrays(ins, outs)
-        probs = (perms.real ** 2 + perms.imag ** 2).reshape(m, S.shape[0])
-        pmf = (probs / norm[None, :]) @ weights
-        self.pmfs[self.current_key] = pmf
+        m = len(keys[0])
+        eye = np.eye(m, dtype=np.int64)
+        outputs = np.asarray(keys, dtype=np.int64)[:, None, :] + eye[None, :, :]   # P x m x m
+        for key, outs_k in zip(keys, outputs):
+            self.possible_outputs[key] = outs_k     # row o: the prefix plus one particle in o
+        P, ns = len(keys), S.shape[0]
+        calc = self._bs_permanent_calculator
+        perms = calc.compute_permanent_outer(S, outputs.reshape(P * m, m)) \
+            if hasattr(calc, "compute_permanent_outer") else None
+        if perms is not None:   # device tensor (P * m) x |S|: pmfs formed on the device
+            T = D.torch()
+            probs = (perms.real ** 2 + perms.imag ** 2) / T.from_numpy(norm).to(perms.device)
+            pmfs = (probs @ T.from_numpy(weights).to(perms.device)).reshape(P, m).cpu().numpy()
+        else:
+            ins = np.tile(S, (P * m, 1))               # item = (prefix * m + o) * |S| + i
+            outs = np.repeat(outputs.reshape(P * m, m), ns, axis=0)
+            perms = calc.compute_permanent_batch_arrays(ins, outs)
+            probs = (perms.real ** 2 + perms.imag ** 2).reshape(P, m, ns)
+            pmfs = (probs / norm[None, None, :]) @ weights
+        for key, pmf in zip(keys, pmfs):
+            self.pmfs[key] = pmf
 
     def _sample_from_latest_pmf(self) -> None:
         """generalized_cliffords_simulation_strategy.py:249-266 (unnormalised pmf)."""
@@ -114,5 +149,5 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
         idx = int(np.searchsorted(np.cumsum(pmf), value, side="right"))
         idx = min(idx, len(pmf) - 1)
         self.current_sample_probability = float(pmf[idx])
-        self.r_sample = list(self.possible_outputs[self.current_key][idx])
+        self.r_sample = [int(x) for x in self.possible_outputs[self.current_key][idx]]
         self.current_key = tuple(self.r_sample)
