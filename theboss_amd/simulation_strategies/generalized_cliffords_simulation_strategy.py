@@ -50,8 +50,14 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
 
     def simulate(self, input_state: Sequence[int], samples_number: int = 1) -> List[Tuple[int, ...]]:
         """All samples advance together, one particle per round: the layers a round needs
-        (one per distinct r-prefix not yet memoised) go to the GPU in one launch. Each
-        sample then draws from its prefix's pmf as the reference does (:249-266)."""
+        (one per distinct r-prefix not yet memoised) go to the GPU in one launch per
+        particle number. Each sample then draws from its prefix's pmf as the reference
+        does (:249-266)."""
+        self._initialize_simulation(input_state)
+        return self._lock_step(max(int(samples_number), 0))
+
+    def _initialize_simulation(self, input_state: Sequence[int]) -> None:
+        """generalized_cliffords_simulation_strategy.py:37-60 (state, substates, memo)."""
         self.input_state = [int(x) for x in as_occupation(input_state)]
         self.number_of_input_photons = sum(self.input_state)
         self._labeled_states = defaultdict(list)
@@ -59,19 +65,39 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
             self._labeled_states[sum(s)].append(s)
         self._layer_cache = {}
         self.pmfs = {}
-        m, S = len(self.input_state), max(int(samples_number), 0)
+
+    def _particles_kept(self, samples: int) -> Optional[np.ndarray]:
+        """Which samples take a particle in this round (None: all of them)."""
+        return None
+
+    def _draw(self, pmfs: np.ndarray) -> np.ndarray:
+        """The reference's draw for a stack of pmfs (:249-266): the first index whose
+        running sum (sequential, as cumsum) exceeds random() * sum(pmf)."""
+        cum = np.cumsum(pmfs, axis=1)
+        value = self._rng.random(len(pmfs)) * cum[:, -1]
+        return np.minimum((cum <= value[:, None]).sum(axis=1), pmfs.shape[1] - 1)
+
+    def _compute_missing(self, keys) -> None:
+        missing = {k for k in keys if k not in self.pmfs}
+        by_count = defaultdict(list)
+        for k in missing:
+            by_count[sum(k)].append(k)
+        for _, group in sorted(by_count.items()):
+            self._calculate_layers(sorted(group))
+
+    def _lock_step(self, S: int) -> List[Tuple[int, ...]]:
+        m = len(self.input_state)
         current = np.zeros((S, m), dtype=np.int64)
         keys = [tuple([0] * m)] * S
         for _ in range(self.number_of_input_photons if S else 0):
-            missing = sorted({k for k in keys if k not in self.pmfs})
-            if missing:
-                self._calculate_layers(missing)
-            # the reference's draw, all samples at once: first index whose running sum
-            # (sequential, as cumsum) exceeds random() * sum(pmf)
-            cum = np.cumsum(np.stack([self.pmfs[k] for k in keys]), axis=1)
-            value = self._rng.random(S) * cum[:, -1]
-            idx = np.minimum((cum <= value[:, None]).sum(axis=1), m - 1)
-            current[np.arange(S), idx] += 1
+            kept = self._particles_kept(S)
+            active = np.arange(S) if kept is None else np.nonzero(kept)[0]
+            if active.size == 0:
+                continue
+            act_keys = [keys[i] for i in active]
+            self._compute_missing(act_keys)
+            idx = self._draw(np.stack([self.pmfs[k] for k in act_keys]))
+            current[active, idx] += 1
             keys = list(map(tuple, current.tolist()))
         if S:
             self.r_sample = list(keys[-1])
@@ -116,15 +142,21 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
         self._calculate_layers([self.current_key])
 
     def _calculate_layers(self, keys: List[Tuple[int, ...]]) -> None:
-        """pmfs of the given r-prefixes (all with the same particle count) from one batched
-        launch of m * |substates| permanents per prefix (:136-171)."""
+        """Memoise the pmfs of the given r-prefixes (all with the same particle count)."""
+        outputs, pmfs = self._layer_pmfs(keys)
+        for key, outs_k, pmf in zip(keys, outputs, pmfs):
+            self.possible_outputs[key] = outs_k     # row o: the prefix plus one particle in o
+            self.pmfs[key] = pmf
+
+    def _layer_pmfs(self, keys: List[Tuple[int, ...]]):
+        """(candidate outputs P x m x m, pmfs P x m) of the given r-prefixes (all with the
+        same particle count) from one batched launch of m * |substates| permanents per
+        prefix (:136-171)."""
         k = sum(keys[0]) + 1
         S, weights, norm = self._layer_constants(k)
         m = len(keys[0])
         eye = np.eye(m, dtype=np.int64)
         outputs = np.asarray(keys, dtype=np.int64)[:, None, :] + eye[None, :, :]   # P x m x m
-        for key, outs_k in zip(keys, outputs):
-            self.possible_outputs[key] = outs_k     # row o: the prefix plus one particle in o
         P, ns = len(keys), S.shape[0]
         calc = self._bs_permanent_calculator
         perms = calc.compute_permanent_outer(S, outputs.reshape(P * m, m)) \
@@ -139,8 +171,7 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
             perms = calc.compute_permanent_batch_arrays(ins, outs)
             probs = (perms.real ** 2 + perms.imag ** 2).reshape(P, m, ns)
             pmfs = (probs / norm[None, None, :]) @ weights
-        for key, pmf in zip(keys, pmfs):
-            self.pmfs[key] = pmf
+        return outputs, pmfs
 
     def _sample_from_latest_pmf(self) -> None:
         """generalized_cliffords_simulation_strategy.py:249-266 (unnormalised pmf)."""
