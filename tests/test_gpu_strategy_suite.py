@@ -48,7 +48,7 @@ def _factory(cfg, strategy):
     return SimulationStrategyFactory(cfg, calc, strategy), calc
 
 
-def _within_bound(exact_calc, strategy_obj, cfg, outcomes):
+def _within_bound(exact_calc, strategy_obj, cfg, outcomes, approximation_bound=0.0):
     from theboss_amd.distribution_calculators import BSSampleBasedDistributionCalculator
     from theboss_amd.quantum_computations_utilities import (
         count_total_variation_distance, count_tv_distance_error_bound_of_experiment_results)
@@ -59,7 +59,7 @@ def _within_bound(exact_calc, strategy_obj, cfg, outcomes):
     tvd = count_total_variation_distance(exact, freq)
     bound = count_tv_distance_error_bound_of_experiment_results(len(outcomes), SAMPLES, P_ERR)
     assert abs(sum(exact) - 1) < 1e-9
-    assert tvd <= bound, (tvd, bound)
+    assert tvd <= approximation_bound + bound, (tvd, approximation_bound, bound)
 
 
 @pytest.mark.parametrize("state", [STANDARD, BINNED], ids=["standard", "binned"])
@@ -93,6 +93,50 @@ def test_uniform_losses_sampling_accuracy(T, state, stype):
     _within_bound(exact, factory.generate_strategy(), cfg, exact.get_outcomes_in_proper_order())
 
 
+@pytest.mark.parametrize("state", [STANDARD, BINNED], ids=["standard", "binned"])
+@pytest.mark.parametrize("lossy", [False, True], ids=["lossless", "lossy"])
+def test_bobs_exact_level_sampling_accuracy(T, state, lossy):
+    """tests/test_bobs_strategy.py:20-54: BOBS with hierarchy_level = m (nothing
+    approximated) samples the exact distribution; lossy runs scale the matrix by
+    sqrt(eta)."""
+    from theboss_amd.distribution_calculators import (BSDistributionCalculatorWithFixedLosses,
+                                                      BSDistributionCalculatorWithUniformLosses)
+    from theboss_amd.simulation_strategies import StrategyType
+    from theboss_amd.states import generate_possible_states
+
+    cfg = _config(state, lossy)
+    cfg.hierarchy_level = cfg.number_of_modes
+    factory, calc = _factory(cfg, StrategyType.BOBS)
+    if lossy:
+        exact = BSDistributionCalculatorWithUniformLosses(cfg, calc)
+        outcomes = exact.get_outcomes_in_proper_order()
+        factory.bs_permanent_calculator.matrix *= pow(0.5, 0.5)
+    else:
+        exact = BSDistributionCalculatorWithFixedLosses(cfg, calc)
+        outcomes = generate_possible_states(sum(state), len(state))
+    _within_bound(exact, factory.generate_strategy(), cfg, outcomes)
+
+
+def test_bobs_approximate_sampling_accuracy(T):
+    """tests/test_bobs_strategy.py:56-90: 3 of 5 modes approximated (hierarchy level 2),
+    eta = 0.5: TVD within the BOBS bound eta^2 (n - h) / 2 + eta (1 - eta) / 2 plus the
+    statistical bound. As in the reference (gcc_based_strategies_tests_base.py:74-86), the
+    exact calculator is built while the configuration holds [1, 1, 1, 1, 0] (it keeps a
+    deep copy) and the strategy then samples the input [0, 0, 3, 1, 0]."""
+    from theboss_amd.distribution_calculators import BSDistributionCalculatorWithUniformLosses
+    from theboss_amd.simulation_strategies import StrategyType
+
+    cfg = _config(STANDARD, lossy=True)
+    cfg.hierarchy_level = 2
+    factory, calc = _factory(cfg, StrategyType.BOBS)
+    exact = BSDistributionCalculatorWithUniformLosses(cfg, calc)
+    cfg.initial_state = [0, 0, 3, 1, 0]
+    factory.bs_permanent_calculator.matrix *= pow(0.5, 0.5)
+    eta, n = 0.5, cfg.initial_number_of_particles
+    bound = eta ** 2 / 2 * (n - cfg.hierarchy_level) + eta * (1 - eta) / 2
+    _within_bound(exact, factory.generate_strategy(), cfg, exact.get_outcomes_in_proper_order(), bound)
+
+
 def test_haar_average_probability(T, oracle):
     """simulation_strategies_tests_common.py:319-368: averaged over Haar matrices, every
     outcome's frequency approaches 1 / (number of outcomes) (the Haar average of the output
@@ -114,3 +158,23 @@ def test_haar_average_probability(T, oracle):
     freq /= freq.sum()
     dim = len(outcomes)
     assert np.abs(freq - 1 / dim).max() < 0.6 / dim
+
+
+@pytest.mark.parametrize("state", [[1, 1, 1, 0], [3, 1, 2, 0]], ids=["standard", "binned"])
+@pytest.mark.parametrize("kind", ["fixed", "uniform"])
+def test_distribution_calculators_sum_to_one(T, oracle, state, kind):
+    """tests/test_distribution_calculators.py:56-99: Haar 4-mode matrix, 2 particles lost
+    (fixed) or eta = 0.5 (uniform), Ryser: the distribution sums to 1."""
+    from theboss_amd.distribution_calculators import (BosonSamplingExperimentConfiguration,
+                                                      BSDistributionCalculatorWithFixedLosses,
+                                                      BSDistributionCalculatorWithUniformLosses)
+    from theboss_amd.permanent_calculators import GPURyserPermanentCalculator
+
+    U = oracle.haar_unitary(4, 300)
+    cfg = BosonSamplingExperimentConfiguration(
+        interferometer_matrix=U, initial_state=state, initial_number_of_particles=sum(state),
+        number_of_modes=4, number_of_particles_lost=2, number_of_particles_left=sum(state) - 2,
+        uniform_transmissivity=0.5)
+    calc = GPURyserPermanentCalculator(U, input_state=state)
+    cls = BSDistributionCalculatorWithFixedLosses if kind == "fixed" else BSDistributionCalculatorWithUniformLosses
+    assert np.isclose(sum(cls(cfg, calc).calculate_distribution()), 1, rtol=0, atol=1e-12)
