@@ -81,6 +81,19 @@ def test_glynn_fp64_closed_forms(T, ops, oracle, n):
         assert abs(g - e) <= REL * abs(e), (n, g, e)
 
 
+def test_glynn_fp64_rank_one_at_n40(T, ops, oracle):
+    """Near the largest supported size (2^39 Gray steps): perm(u v^T) = n! prod u prod v
+    with unit-modulus entries, whose Glynn terms stay far below the result."""
+    import math
+
+    n = 40
+    rng = np.random.default_rng(40)
+    u, v = np.exp(2j * np.pi * rng.random(n)), np.exp(2j * np.pi * rng.random(n))
+    got = ops.glynn_batch(dev_c(T, np.outer(u, v)[None])).cpu().numpy()[0]
+    expect = math.factorial(n) * np.prod(u) * np.prod(v)
+    assert abs(got - expect) <= REL * abs(expect), (got, expect)
+
+
 def test_glynn_fp32_colsum_error(T, ops, oracle):
     from theboss_amd import _lib
 
@@ -219,6 +232,38 @@ def test_submatrices_collision_inputs_vs_oracle(T, oracle):
         got = np.asarray(GPUBSCCRyserSubmatricesPermanentCalculator(U, s, t).compute_permanents())
         ref = oracle.cc_submatrices(U, s, t)
         assert np.max(np.abs(got - ref)) <= 1e-12, (s, t)
+
+
+@pytest.mark.parametrize("k", [22, 26, 32])
+def test_sweep_large_k_laplace_vs_glynn(T, ops, oracle, k):
+    """k > 21 takes particle 1's column from SGPRs. For every output row o of a random
+    k x k matrix A: sum_nu A[o][nu] minor_nu(A without row o) = perm(A), with perm(A) from
+    the Glynn kernel (and from the oracle at k = 22)."""
+    A = oracle.haar_unitary(2 * k, 900 + k)[:k, :k]
+    Vts = np.stack([np.delete(A, o, axis=0).T for o in (0, k // 2, k - 1)])  # particle-major
+    minors = ops.sweep_minors_batch(dev_c(T, Vts), k).cpu().numpy()
+    perm = ops.glynn_batch(dev_c(T, A[None])).cpu().numpy()[0]
+    if k == 22:
+        ref = oracle.perm_glynn_matrix(A, 1)
+        assert abs(perm - ref) <= REL * abs(ref)
+    for row, o in enumerate((0, k // 2, k - 1)):
+        lap = np.sum(A[o, :] * minors[row])
+        assert abs(lap - perm) <= REL * abs(perm), (k, o, lap, perm)
+
+
+def test_sweep_rank_one_at_max_k(T, ops):
+    """The largest sweep, k = TB_SWEEP_MAX_K = 40: for V[nu][j] = a_nu b_j,
+    minor_nu = (k-1)! prod_{mu != nu} a_mu prod_j b_j."""
+    import math
+
+    from theboss_amd import _lib
+
+    k = _lib.TB_SWEEP_MAX_K
+    rng = np.random.default_rng(k)
+    a, b = np.exp(2j * np.pi * rng.random(k)), np.exp(2j * np.pi * rng.random(k - 1))
+    minors = ops.sweep_minors_batch(dev_c(T, np.outer(a, b)[None]), k).cpu().numpy()[0]
+    expect = math.factorial(k - 1) * np.prod(a) / a * np.prod(b)
+    assert np.max(np.abs(minors - expect) / np.abs(expect)) <= REL
 
 
 def test_sweep_k20_laplace_identity(T, ops, oracle):
