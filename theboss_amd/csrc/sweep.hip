@@ -36,13 +36,14 @@
 #define TB_STEP_FENCE() __builtin_amdgcn_sched_barrier(0)
 #endif
 // Column of particle 1 (flipped on half of all Gray steps): in VGPRs for
-// k <= TB_SWEEP_R0_MAX_K (register budget), above that its first TB_SWEEP_SROW values in
-// SGPRs (scalar operands of the adds) and the rest from LDS.
+// k <= TB_SWEEP_R0_MAX_K, above that its first TB_SWEEP_SROW values in SGPRs (scalar
+// operands of the adds) and the rest from LDS. The SGPR form leaves the VGPRs that let
+// k = 17..21 load the next flip's column group one step ahead without spilling.
 #ifndef TB_SWEEP_R0
 #define TB_SWEEP_R0 1
 #endif
 #ifndef TB_SWEEP_R0_MAX_K
-#define TB_SWEEP_R0_MAX_K 21
+#define TB_SWEEP_R0_MAX_K 16
 #endif
 #ifndef TB_SWEEP_R0_LARGE
 #define TB_SWEEP_R0_LARGE 2
@@ -50,6 +51,7 @@
 #ifndef TB_SWEEP_SROW
 #define TB_SWEEP_SROW 20
 #endif
+
 
 namespace tb {
 
@@ -60,15 +62,17 @@ constexpr int kSweepU = TB_SWEEP_U;  // Gray bits unrolled per block (static fli
 constexpr int kSweepMaxC = 16;  // largest chunk (bits); sizes the LDS corrector store
 
 // h += sg * column (sg = +-1 per lane, or a compile-time sign).
-template <int R, int MODE>
+constexpr int kSweepG = 4;  // column values per LDS group
+
+template <int R, int MODE, bool PRE = false>
 __device__ __forceinline__ void sweep_flip(double (&hr)[R], double (&hi)[R], const cd *col,
-                                           double sg) {
-  constexpr int G = 4;
+                                           double sg, const cd *pre = nullptr) {
+  constexpr int G = kSweepG;
   constexpr int NG = (R + G - 1) / G;
   cd cur[G], nxt[G];
 #pragma unroll
   for (int e = 0; e < G; ++e)
-    if (e < R) cur[e] = col[e];
+    if (e < R) cur[e] = PRE ? pre[e] : col[e];
 #pragma unroll
   for (int g = 0; g < NG; ++g) {
     if (g + 1 < NG) {
@@ -117,9 +121,10 @@ struct Col0 {
   SRow<S, double> sr;
 };
 
-template <int R, int MODE, int R0>
+template <int R, int MODE, int R0, bool PRE = false>
 __device__ __forceinline__ void sweep_flip_col(double (&hr)[R], double (&hi)[R], const cd *Vs,
-                                               const Col0<R, R0> &c0, int i, double sg) {
+                                               const Col0<R, R0> &c0, int i, double sg,
+                                               const cd *pre = nullptr) {
   if constexpr (R0 == 2) {
     if (i == 1) {  // particle 1 (compile-time at every static flip site)
       flip_srow<R, Col0<R, R0>::S, MODE, double>(hr, hi, c0.sr,
@@ -145,7 +150,7 @@ __device__ __forceinline__ void sweep_flip_col(double (&hr)[R], double (&hi)[R],
       return;
     }
   }
-  sweep_flip<R, MODE>(hr, hi, Vs + i * R, sg);
+  sweep_flip<R, MODE, PRE>(hr, hi, Vs + i * R, sg, pre);
 }
 
 template <int K, int U, int R0>
@@ -192,6 +197,15 @@ __device__ __forceinline__ void sweep_chunk(const cd *Vs, const Col0<K - 1, R0> 
   // Flip of bit i at local step l: the new bit is 1 ^ t_{i+1}. t_{i+1} == 0: delta goes
   // +1 -> -1 (h -= V, corrector += 2T); t_{i+1} == 1: delta goes -1 -> +1 (h += V,
   // corrector -= 2T).
+  // Particle 1 (odd local steps) is not read from LDS, so an odd step issues the LDS loads
+  // of the next step's first column group; they complete under this step's product.
+  constexpr bool kPre = R0 != 0;
+  cd pre[kSweepG];
+  auto prefetch = [&](const cd *col) {
+#pragma unroll
+    for (int e = 0; e < kSweepG; ++e)
+      if (e < R) pre[e] = col[e];
+  };
   auto static_steps = [&](int blk) {
 #pragma unroll
     for (int j = 1; j < (1 << U); ++j) {
@@ -200,18 +214,28 @@ __device__ __forceinline__ void sweep_chunk(const cd *Vs, const Col0<K - 1, R0> 
         if ((j >> (i + 1)) & 1) {
           Esr[i] -= 2.0 * Tr;
           Esi[i] -= 2.0 * Ti;
-          sweep_flip_col<R, 1, R0>(hr, hi, Vs, c1, i + 1, 1.0);
+          sweep_flip_col<R, 1, R0, kPre>(hr, hi, Vs, c1, i + 1, 1.0, pre);
         } else {
           Esr[i] += 2.0 * Tr;
           Esi[i] += 2.0 * Ti;
-          sweep_flip_col<R, -1, R0>(hr, hi, Vs, c1, i + 1, -1.0);
+          sweep_flip_col<R, -1, R0, kPre>(hr, hi, Vs, c1, i + 1, -1.0, pre);
         }
       } else {
         const bool t_next = (U < c) ? ((blk & 1) != 0) : q_odd;
         const double sg = t_next ? 1.0 : -1.0;
         Esr[i] = fma(-2.0 * sg, Tr, Esr[i]);
         Esi[i] = fma(-2.0 * sg, Ti, Esi[i]);
-        sweep_flip_col<R, 0, R0>(hr, hi, Vs, c1, i + 1, sg);
+        sweep_flip_col<R, 0, R0, kPre>(hr, hi, Vs, c1, i + 1, sg, pre);
+      }
+      if constexpr (kPre) {
+        if (j & 1) {
+          if (j + 1 < (1 << U)) {
+            prefetch(Vs + (__builtin_ctz(j + 1) + 1) * R);
+          } else {  // the next block's first flip (particle 1's column past the last block)
+            const int inext = (blk + 1 < nblocks) ? U + __builtin_ctz(blk + 1) : 0;
+            prefetch(Vs + (inext + 1) * R);
+          }
+        }
       }
       const cd p = sweep_prod<R>(hr, hi);
       if (j & 1) {
@@ -234,7 +258,7 @@ __device__ __forceinline__ void sweep_chunk(const cd *Vs, const Col0<K - 1, R0> 
     e.re = fma(-2.0 * sg, Tr, e.re);
     e.im = fma(-2.0 * sg, Ti, e.im);
     Ed[(i - U) * 64] = e;
-    sweep_flip<R, 0>(hr, hi, Vs + (i + 1) * R, sg);
+    sweep_flip<R, 0, kPre>(hr, hi, Vs + (i + 1) * R, sg, pre);
     {
       const cd p = sweep_prod<R>(hr, hi);
       Tr += p.re;
