@@ -200,7 +200,7 @@ def main() -> int:
         "fp64_flops_per_sample": sample_flops(N, M),
         "fp64_tflops_sampler": batched * sample_flops(N, M) / 1e12,
         "roofline": {
-            "kernel": "sweep_kernel<20,3,2> (C&C minors sweep, k=20, 4096 items, chunk 2^%d)"
+            "kernel": "sweep_kernel<20,3,3> (C&C minors sweep, k=20, 4096 items, chunk 2^%d)"
                       % (tuned["chunk_bits"].get(N, 11) if tuned else 11),
             "flops_per_launch": SAMPLES * sweep_flops(k),
             "bound": "valu",

@@ -192,11 +192,12 @@ def test_uniform_loss_strategy_matches_exact_distribution(T, oracle):
     inp = [1, 1, 1, 0]
     strat = GeneralizedCliffordsBUniformLossesSimulationStrategy(GPURyserPermanentCalculator(U), eta, seed=3)
     samples = strat.simulate(inp, S)
+    assert all(isinstance(s, np.ndarray) and s.dtype == np.int64 for s in samples)  # as the reference
     states = oracle.generate_possible_states(3, m, losses=True)
     idx = {st: i for i, st in enumerate(states)}
     freq = np.zeros(len(states))
     for s in samples:
-        freq[idx[s]] += 1
+        freq[idx[tuple(int(x) for x in s)]] += 1
     freq /= S
     exact = np.array(oracle.uniform_losses_distribution(U, inp, eta, oracle.perm_ryser))
     assert 0.5 * np.abs(freq - exact).sum() < oracle.tvd_error_bound(len(states), S, 1e-4)

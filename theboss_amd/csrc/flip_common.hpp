@@ -1,7 +1,8 @@
 #pragma once
-// Gray-walk building blocks shared by the Glynn and gradient kernels: complex values of
-// either precision, balanced product trees, and the flip of one matrix row into the
-// half column sums (from LDS/global rows, VGPR rows or SGPR-held values).
+// Gray-walk building blocks shared by the Glynn, sweep and gradient kernels: complex values
+// of either precision, balanced product trees, and the flip of one matrix row into the
+// half column sums (from LDS/global rows, VGPR rows, SGPR-held values or broadcast-form
+// VGPR vectors).
 #include "tb_common.hpp"
 
 namespace tb {
@@ -149,6 +150,62 @@ __device__ __forceinline__ void load_srow(SRow<S, F> &sr, P row) {
     sr.re[j] = uniform_value(v.re);
     sr.im[j] = uniform_value(v.im);
   }
+}
+
+// A row (or column) of N complex values in broadcast form: value t (re/im interleaved)
+// sits in lane t mod 16 of every 16-lane row of VGPR pair t / 16, i.e. 2 ceil(2N/16) VGPRs
+// per vector. A flip adds it to the half sums with one v_fmac_f64 per value whose first
+// operand is taken from that lane by DPP row_newbcast, with no LDS read. Every lane of
+// the wave must be active at a flip (DPP reads other lanes' registers).
+template <int N>
+struct Bcast {
+  static constexpr int P = (2 * N + 15) / 16;
+  double v[P];
+};
+
+// Lane `lane` of the wave loads its slots of the vector `src` (N complex values).
+template <int N>
+__device__ __forceinline__ void load_bcast(Bcast<N> &b, const double *src, int lane) {
+#pragma unroll
+  for (int p = 0; p < Bcast<N>::P; ++p) {
+    const int t = 16 * p + (lane & 15);
+    b.v[p] = t < 2 * N ? src[t] : 0.0;
+  }
+}
+
+// h += s * v[L] (L = lane within each 16-lane row), and the same with -v[L].
+template <int L>
+__device__ __forceinline__ void fmac_bcast(double &h, double v, double s) {
+  asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+      : "+v"(h)
+      : "v"(v), "v"(s), "i"(L));
+}
+template <int L>
+__device__ __forceinline__ void fmac_bcast_neg(double &h, double v, double s) {
+  asm("v_fmac_f64_dpp %0, -%1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+      : "+v"(h)
+      : "v"(v), "v"(s), "i"(L));
+}
+
+// h += sg * vector (MODE 0), h += vector (MODE 1, s = 1) or h -= vector (MODE -1, s = 1).
+template <int T, int N, int MODE>
+__device__ __forceinline__ void flip_bcast(double (&hr)[N], double (&hi)[N], const Bcast<N> &b,
+                                           double s) {
+  if constexpr (T < 2 * N) {
+    double &h = (T & 1) ? hi[T / 2] : hr[T / 2];
+    if constexpr (MODE == -1)
+      fmac_bcast_neg<T % 16>(h, b.v[T / 16], s);
+    else
+      fmac_bcast<T % 16>(h, b.v[T / 16], s);
+    flip_bcast<T + 1, N, MODE>(hr, hi, b, s);
+  }
+}
+
+// 1.0 in a VGPR the compiler cannot fold away (second operand of fixed-direction flips).
+__device__ __forceinline__ double vgpr_one() {
+  double one = 1.0;
+  asm volatile("" : "+v"(one));
+  return one;
 }
 
 }  // namespace tb

@@ -14,7 +14,10 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t tbgl_partial_bytes(int n, int batch, int c_pref) {
   if (n < tbgl::kWideMinN) return 0;
-  return align256((size_t)batch * tbgl::glynn_plan(n, batch, c_pref).wpm * sizeof(tb::cd));
+  // large enough for the plan of either precision (their static bits may differ)
+  const int wpm = std::max(tbgl::glynn_plan(n, batch, c_pref, 2).wpm,
+                           tbgl::glynn_plan(n, batch, c_pref, tbgl::glynn_u_max(n)).wpm);
+  return align256((size_t)batch * wpm * sizeof(tb::cd));
 }
 
 size_t glynn_workspace_bytes(int n, int batch, int precision, int c_pref) {

@@ -11,8 +11,9 @@
 // owns one chunk: it seeds its column sums at the chunk start in O(n^2) and then walks
 // the chunk with O(n) updates per term. Because chunks are aligned, the row flipped at
 // local step l is ctz(l) for every lane, so in WIDE mode (all 64 lanes of a wave on one
-// matrix) the flipped row is wave-uniform: it is read from LDS as a broadcast, held in
-// VGPRs (row 0, when registers allow) or partly in SGPRs (row 0 otherwise).
+// matrix) the flipped row is wave-uniform. FP64 keeps the most frequently flipped rows
+// (all static ones) in broadcast-form VGPRs (one DPP v_fmac_f64 per value, no LDS read)
+// and reads the others from LDS as a broadcast; FP32 holds row 0 in VGPRs.
 //
 // Half column sums h_j = s_j / 2 are carried so a flip is a plain add/sub of M[i][j];
 // perm = 2 * sum_t sign(t) prod_j h_j(t). sign(t) = (-1)^(popcount gray(t)) = (-1)^(t & 1).
@@ -21,6 +22,11 @@
 #include "flip_common.hpp"
 
 namespace tb {
+
+// FP64 wide kernels keep rows 0..U+1 in broadcast form up to this n, rows 0..U-1 above.
+#ifndef TB_GLYNN_BCAST_WIDE_MAX_N
+#define TB_GLYNN_BCAST_WIDE_MAX_N 24
+#endif
 
 // Occupancy target for the wide kernels: 2 waves per SIMD (<= 256 VGPRs).
 #ifndef TB_WIDE_ATTR
@@ -40,9 +46,26 @@ struct RegRows {
   F re[R0 > 0 ? R0 : 1][N], im[R0 > 0 ? R0 : 1][N];
 };
 
-template <int N, int R0, int S, int MODE, typename F, typename P>
+// Rows 0..B-1 in broadcast form (FP64 only): every static flip and 3/4 of the
+// block-start flips read no LDS.
+template <int N, int B>
+struct BcastRows {
+  Bcast<N> r[B > 0 ? B : 1];
+  double one;
+};
+
+template <int N, int R0, int S, int MODE, typename F, typename P, int B = 0>
 __device__ __forceinline__ void flip_row(F (&hr)[N], F (&hi)[N], const RegRows<N, R0, F> &rr,
-                                         const SRow<S, F> &sr, P M, int i, F sg) {
+                                         const SRow<S, F> &sr, P M, int i, F sg,
+                                         const BcastRows<N, B> *bs = nullptr) {
+  if constexpr (B > 0) {
+    if (i < B) {  // compile-time at the static flip sites, uniform at block starts
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+        if (u == i) flip_bcast<0, N, MODE>(hr, hi, bs->r[u], MODE == 0 ? sg : bs->one);
+      return;
+    }
+  }
   if constexpr (S > 0) {
     if (i == 0) {
       flip_srow<N, S, MODE, F>(hr, hi, sr, M, sg);
@@ -64,13 +87,21 @@ __device__ __forceinline__ void flip_row(F (&hr)[N], F (&hi)[N], const RegRows<N
 // One lane walks `iters` chunks q = q0 + it*qstride of 2^C Gray ranks each; M is the
 // n x n matrix as complex rows. `topbit_fixed` means C == N-1 (the whole enumeration in
 // one chunk), so the Gray bit above the chunk is 0.
-template <int N, int U, int R0, typename F, typename P, int S = 0>
+template <int N, int U, int R0, typename F, typename P, int S = 0, int B = 0>
 __device__ __forceinline__ void glynn_lane(P M, int C, uint64_t q0, uint64_t qstride, int iters,
                                            double &acc_re, double &acc_im) {
   constexpr int NB = N - 1;  // Gray bits
   const bool topbit_fixed = (C == NB);
   SRow<S, F> sr;
   if constexpr (S > 0) load_srow(sr, M);
+  BcastRows<N, B> bs;
+  if constexpr (B > 0) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int u = 0; u < B; ++u) load_bcast(bs.r[u], (const double *)(M + (size_t)u * N), lane);
+    bs.one = vgpr_one();
+    asm volatile("s_nop 4");  // VALU writes of the store before its first DPP read
+  }
   RegRows<N, R0, F> rr;
   if constexpr (R0 > 0) {
 #pragma unroll
@@ -124,16 +155,16 @@ __device__ __forceinline__ void glynn_lane(P M, int C, uint64_t q0, uint64_t qst
         if (i + 1 < U) {
           // Direction fixed by the local bit i+1 of j.
           if ((j >> (i + 1)) & 1)
-            flip_row<N, R0, S, 1, F>(hr, hi, rr, sr, M, i, F(1));
+            flip_row<N, R0, S, 1, F, P, B>(hr, hi, rr, sr, M, i, F(1), &bs);
           else
-            flip_row<N, R0, S, -1, F>(hr, hi, rr, sr, M, i, F(-1));
+            flip_row<N, R0, S, -1, F, P, B>(hr, hi, rr, sr, M, i, F(-1), &bs);
         } else {
           bool add;
           if (U < C)
             add = (blk & 1) != 0;
           else
             add = topbit_fixed ? false : q_odd;
-          flip_row<N, R0, S, 0, F>(hr, hi, rr, sr, M, i, add ? F(1) : F(-1));
+          flip_row<N, R0, S, 0, F, P, B>(hr, hi, rr, sr, M, i, add ? F(1) : F(-1), &bs);
         }
         cpx<F> p = TreeProdT<F, 0, N>::run(hr, hi);
         if (j & 1) {
@@ -155,7 +186,14 @@ __device__ __forceinline__ void glynn_lane(P M, int C, uint64_t q0, uint64_t qst
         add = ((blk >> (i + 1 - U)) & 1) != 0;
       else
         add = topbit_fixed ? false : q_odd;
-      flip_mem<N, 0, F>(hr, hi, M + (size_t)i * N, add ? F(1) : F(-1));
+      if constexpr (B > U) {
+        if (i < B)
+          flip_row<N, 0, 0, 0, F, P, B>(hr, hi, rr, sr, M, i, add ? F(1) : F(-1), &bs);
+        else
+          flip_mem<N, 0, F>(hr, hi, M + (size_t)i * N, add ? F(1) : F(-1));
+      } else {
+        flip_mem<N, 0, F>(hr, hi, M + (size_t)i * N, add ? F(1) : F(-1));
+      }
       {
         cpx<F> p = TreeProdT<F, 0, N>::run(hr, hi);
         cre += (double)p.re;
@@ -185,8 +223,12 @@ __global__ __launch_bounds__(256) TB_WIDE_ATTR(N) void glynn_wide_kernel(const c
   for (int e = threadIdx.x; e < N * N; e += 256) Ms[e] = src[e];
   __syncthreads();
   double are = 0.0, aim = 0.0;
+  // R0 = 0: row 0's first values in SGPRs; 1: row 0 in VGPRs; 2 (FP64): rows 0..U+1
+  // (n <= TB_GLYNN_BCAST_WIDE_MAX_N) or 0..U-1 in broadcast form.
   constexpr int S = (R0 == 0) ? TB_GLYNN_SROW : 0;
-  glynn_lane<N, U, R0, F, const cpx<F> *, (S < N ? S : N)>(
+  constexpr int BW = (N <= TB_GLYNN_BCAST_WIDE_MAX_N) ? U + 2 : U;  // rows in broadcast form
+  constexpr int B = (R0 == 2) ? (BW < N - 1 ? BW : N - 1) : 0;
+  glynn_lane<N, U, (R0 == 2 ? 0 : R0), F, const cpx<F> *, (S < N ? S : N), B>(
       (const cpx<F> *)Ms, C, (uint64_t)wq * 64 + lane, (uint64_t)wpm * 64, iters, are, aim);
   are = wave_sum(are);
   aim = wave_sum(aim);
@@ -259,10 +301,25 @@ namespace tbgl {
 #ifndef TB_GLYNN_F64_R0_MAX_N
 #define TB_GLYNN_F64_R0_MAX_N 24
 #endif
+#ifndef TB_GLYNN_F64_BCAST_MAX_N
+#define TB_GLYNN_F64_BCAST_MAX_N 48
+#endif
 template <int N, typename F>
 constexpr int glynn_reg_rows() {
+  if (sizeof(F) == 8 && N <= TB_GLYNN_F64_BCAST_MAX_N) return 2;
   return (sizeof(F) == 4 || N <= TB_GLYNN_F64_R0_MAX_N) ? 1 : 0;
 }
+
+// Static Gray bits (unrolled block of 2^U steps) of the WIDE kernels: FP64 at n >= 25,
+// whose static rows are all in broadcast form, unrolls TB_GLYNN_U_LARGE bits.
+#ifndef TB_GLYNN_U_LARGE
+#define TB_GLYNN_U_LARGE 3
+#endif
+template <int N, typename F>
+constexpr int glynn_u() {
+  return (sizeof(F) == 8 && N >= 25) ? TB_GLYNN_U_LARGE : 2;
+}
+inline int glynn_u_max(int n) { return n >= 25 ? std::max(2, TB_GLYNN_U_LARGE) : 2; }
 
 constexpr int kWideMinN = 11;  // below this a lane owns a whole matrix
 constexpr int kMaxC = 12;      // Gray bits per lane chunk
@@ -277,7 +334,7 @@ struct GlynnPlan {
 // mid-size matrices still fill the GPU; C <= 12 bounds a lane's walk between re-seeds.
 // A tuned C (c_pref > 0) replaces that choice. Always: C >= U and a matrix gets at
 // least 4 waves (one workgroup). Re-seeding costs n^2 per 2^C steps.
-inline GlynnPlan glynn_plan(int n, int batch, int c_pref) {
+inline GlynnPlan glynn_plan(int n, int batch, int c_pref, int u = 2) {
   GlynnPlan p{};
   const int nb = n - 1;
   int C;
@@ -290,7 +347,7 @@ inline GlynnPlan glynn_plan(int n, int batch, int c_pref) {
     C = std::max(C, std::min(7, nb - 8));
   }
   C = std::min(C, kMaxC);
-  C = std::max(C, 2);
+  C = std::max(C, u);
   C = std::min(C, nb - 8);
   p.C = C;
   const int wave_bits = nb - p.C - 6;  // log2(waves needed at one chunk per lane), >= 2
@@ -311,8 +368,8 @@ inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, int batch, t
     (void)c_pref;
     return tbh::check_hip(hipGetLastError(), "glynn_narrow_kernel");
   } else {
-    const GlynnPlan p = glynn_plan(N, batch, c_pref);
-    constexpr int U = 2;
+    constexpr int U = glynn_u<N, F>();
+    const GlynnPlan p = glynn_plan(N, batch, c_pref, U);
     const unsigned blocks = (unsigned)batch * (unsigned)(p.wpm / 4);
     hipLaunchKernelGGL((tb::glynn_wide_kernel<N, U, glynn_reg_rows<N, F>(), F>), dim3(blocks), dim3(256), 0, s,
                        A, mstride, p.C, p.wpm, p.iters, partial);

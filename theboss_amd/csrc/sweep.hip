@@ -18,7 +18,8 @@
 //
 // Work split (same skeleton as the Glynn kernel): aligned chunks of 2^c Gray ranks, one
 // per lane; all lanes of a wave belong to one item, so the flipped particle at local
-// step l is 1 + ctz(l) for the whole wave and its column is a broadcast LDS read.
+// step l is 1 + ctz(l) for the whole wave: its column is wave-uniform, held in
+// broadcast-form VGPRs for the most frequent particles and read from LDS otherwise.
 //
 // All k minors without a per-step O(k) update: keep the running chunk sum
 // T = sum (prod delta) prod h and, per in-chunk bit b, a corrector E_b updated only when
@@ -35,18 +36,16 @@
 #ifndef TB_STEP_FENCE
 #define TB_STEP_FENCE() __builtin_amdgcn_sched_barrier(0)
 #endif
-// Column of particle 1 (flipped on half of all Gray steps): in VGPRs for
-// k <= TB_SWEEP_R0_MAX_K, above that its first TB_SWEEP_SROW values in SGPRs (scalar
-// operands of the adds) and the rest from LDS. The SGPR form leaves the VGPRs that let
-// k = 17..21 load the next flip's column group one step ahead without spilling.
-#ifndef TB_SWEEP_R0
-#define TB_SWEEP_R0 1
-#endif
-#ifndef TB_SWEEP_R0_MAX_K
-#define TB_SWEEP_R0_MAX_K 16
-#endif
-#ifndef TB_SWEEP_R0_LARGE
-#define TB_SWEEP_R0_LARGE 2
+// Where the flipped columns come from (R0 mode of the kernel):
+//  3 (k <= TB_SWEEP_BCAST_MAX_K): particles 1..5 in broadcast form in VGPRs (every static
+//    flip and 3/4 of the block-start flips, no LDS read); the rest from LDS;
+//  2 (larger k, where 5 broadcast columns no longer fit 2 waves per SIMD): the first
+//    TB_SWEEP_SROW values of particle 1's column in SGPRs (scalar operands of the adds),
+//    everything else from LDS.
+// LDS flips load their first column group one step ahead (issued by a step that reads no
+// LDS, under its product).
+#ifndef TB_SWEEP_BCAST_MAX_K
+#define TB_SWEEP_BCAST_MAX_K 31
 #endif
 #ifndef TB_SWEEP_SROW
 #define TB_SWEEP_SROW 20
@@ -111,42 +110,32 @@ __device__ __forceinline__ cd sweep_prod(const double (&hr)[R], const double (&h
 // major).
 // Ed: this lane's LDS corrector slots for bits U..c-1 (stride 64 cd).
 // Returns the chunk's contribution to all K minors in mi[].
-// Column of particle 1 (Gray bit 0), held in VGPRs when R0 == 1: it flips on half of all
-// Gray steps, so this halves the LDS broadcast traffic.
-// R0 = 1: the whole column in VGPRs; R0 = 2: its first S values in SGPRs; R0 = 0: none.
+// Register-held column data of one item (R0 mode above).
 template <int R, int R0>
 struct Col0 {
   static constexpr int S = (R0 == 2) ? (TB_SWEEP_SROW < R ? TB_SWEEP_SROW : R) : 0;
-  double re[R0 == 1 ? R : 1], im[R0 == 1 ? R : 1];
+  static constexpr int NU = (R0 == 3) ? (kSweepU + 2 < R ? kSweepU + 2 : R) : 1;
   SRow<S, double> sr;
+  Bcast<R> b[NU];  // particles 1..NU
+  double one;      // 1.0 in a VGPR (second operand of the fixed-direction flips)
 };
 
 template <int R, int MODE, int R0, bool PRE = false>
 __device__ __forceinline__ void sweep_flip_col(double (&hr)[R], double (&hi)[R], const cd *Vs,
                                                const Col0<R, R0> &c0, int i, double sg,
                                                const cd *pre = nullptr) {
+  if constexpr (R0 == 3) {
+    if (i >= 1 && i <= Col0<R, R0>::NU) {  // compile-time at every static flip site
+#pragma unroll
+      for (int u = 0; u < Col0<R, R0>::NU; ++u)
+        if (u == i - 1) flip_bcast<0, R, MODE>(hr, hi, c0.b[u], MODE == 0 ? sg : c0.one);
+      return;
+    }
+  }
   if constexpr (R0 == 2) {
     if (i == 1) {  // particle 1 (compile-time at every static flip site)
       flip_srow<R, Col0<R, R0>::S, MODE, double>(hr, hi, c0.sr,
                                                  (const cpx<double> *)(Vs + R), sg);
-      return;
-    }
-  }
-  if constexpr (R0 == 1) {
-    if (i == 1) {  // particle 1 (compile-time at every static flip site)
-#pragma unroll
-      for (int j = 0; j < R; ++j) {
-        if constexpr (MODE == 1) {
-          hr[j] += c0.re[j];
-          hi[j] += c0.im[j];
-        } else if constexpr (MODE == -1) {
-          hr[j] -= c0.re[j];
-          hi[j] -= c0.im[j];
-        } else {
-          hr[j] = fma(sg, c0.re[j], hr[j]);
-          hi[j] = fma(sg, c0.im[j], hi[j]);
-        }
-      }
       return;
     }
   }
@@ -199,7 +188,8 @@ __device__ __forceinline__ void sweep_chunk(const cd *Vs, const Col0<K - 1, R0> 
   // corrector -= 2T).
   // Particle 1 (odd local steps) is not read from LDS, so an odd step issues the LDS loads
   // of the next step's first column group; they complete under this step's product.
-  constexpr bool kPre = R0 != 0;
+  constexpr bool kPre = true;
+  constexpr bool kStaticLds = R0 != 3;  // static flips of particles >= 2 read LDS
   cd pre[kSweepG];
   auto prefetch = [&](const cd *col) {
 #pragma unroll
@@ -228,7 +218,7 @@ __device__ __forceinline__ void sweep_chunk(const cd *Vs, const Col0<K - 1, R0> 
         sweep_flip_col<R, 0, R0, kPre>(hr, hi, Vs, c1, i + 1, sg, pre);
       }
       if constexpr (kPre) {
-        if (j & 1) {
+        if ((j & 1) && (kStaticLds || j + 1 == (1 << U))) {
           if (j + 1 < (1 << U)) {
             prefetch(Vs + (__builtin_ctz(j + 1) + 1) * R);
           } else {  // the next block's first flip (particle 1's column past the last block)
@@ -258,7 +248,20 @@ __device__ __forceinline__ void sweep_chunk(const cd *Vs, const Col0<K - 1, R0> 
     e.re = fma(-2.0 * sg, Tr, e.re);
     e.im = fma(-2.0 * sg, Ti, e.im);
     Ed[(i - U) * 64] = e;
-    sweep_flip<R, 0, kPre>(hr, hi, Vs + (i + 1) * R, sg, pre);
+    if constexpr (R0 == 3) {
+      // particles U+1 .. NU (3/4 of the block starts) from the broadcast store
+      bool done = false;
+#pragma unroll
+      for (int u = U; u < Col0<R, R0>::NU; ++u) {
+        if (i + 1 == u + 1) {
+          flip_bcast<0, R, 0>(hr, hi, c1.b[u], sg);
+          done = true;
+        }
+      }
+      if (!done) sweep_flip<R, 0, kPre>(hr, hi, Vs + (i + 1) * R, sg, pre);
+    } else {
+      sweep_flip<R, 0, kPre>(hr, hi, Vs + (i + 1) * R, sg, pre);
+    }
     {
       const cd p = sweep_prod<R>(hr, hi);
       Tr += p.re;
@@ -330,21 +333,24 @@ __global__ __launch_bounds__(64) TB_SWEEP_ATTR(K) void sweep_kernel(
     }
     __syncthreads();
     Col0<R, R0> c1;
-    if constexpr (R0 == 1) {
-#pragma unroll
-      for (int j = 0; j < R; ++j) {
-        c1.re[j] = Vs[R + j].re;
-        c1.im[j] = Vs[R + j].im;
-      }
-    }
     if constexpr (R0 == 2) load_srow(c1.sr, (const cpx<double> *)(Vs + R));
+    if constexpr (R0 == 3) {
+#pragma unroll
+      for (int u = 0; u < Col0<R, R0>::NU; ++u) {
+        load_bcast(c1.b[u], (const double *)(Vs + (u + 1) * R), lane);
+      }
+      c1.one = vgpr_one();
+      asm volatile("s_nop 4");  // VALU writes of the store before its first DPP read
+    }
     cd acc = {0.0, 0.0};
     for (int it = 0; it < iters; ++it) {
       const uint64_t q = ((uint64_t)w + (uint64_t)it * wpi) * 64 + lane;
       cd mi[K];
-      if (q < nchunks) {
-        sweep_chunk<K, U, R0>(Vs, c1, Ed + lane, c, q, mi);
-      } else {
+      // Lanes past the last chunk walk a valid one and drop it, so every lane stays active
+      // (the broadcast flips read other lanes' registers).
+      const bool live = q < nchunks;
+      sweep_chunk<K, U, R0>(Vs, c1, Ed + lane, c, live ? q : (q & (nchunks - 1)), mi);
+      if (!live) {
 #pragma unroll
         for (int i = 0; i < K; ++i) mi[i] = {0.0, 0.0};
       }
@@ -422,7 +428,7 @@ int32_t launch_sweep_k(const tb::cd *Vt, int ldv, int64_t item_stride, const int
   // Persistent grid: enough one-wave blocks to fill every SIMD a few times over.
   const int64_t tasks = (int64_t)items * p.wpi;
   const unsigned grid = (unsigned)std::min<int64_t>(tasks, tbs::kSweepMaxBlocks);
-  constexpr int R0 = (K <= TB_SWEEP_R0_MAX_K) ? TB_SWEEP_R0 : TB_SWEEP_R0_LARGE;
+  constexpr int R0 = (K <= TB_SWEEP_BCAST_MAX_K) ? 3 : 2;
   constexpr int U = (K - 1 < tb::kSweepU) ? K - 1 : tb::kSweepU;
   hipLaunchKernelGGL((tb::sweep_kernel<K, U, R0>), dim3(grid), dim3(64), sweep_lds_bytes(K, p.c), s,
                      Vt, ldv, item_stride, item_map, item_count, items, p.c, p.wpi, p.iters,

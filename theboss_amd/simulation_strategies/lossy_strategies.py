@@ -24,6 +24,10 @@ class GeneralizedCliffordsBUniformLossesSimulationStrategy(BatchedGCCBSimulation
         super().__init__(bs_permanent_calculator, transmissivity=transmissivity, seed=seed,
                          chunk_bits=chunk_bits)
 
+    def simulate(self, input_state: Sequence[int], samples_number: int = 1) -> List[np.ndarray]:
+        """Samples as int64 arrays, as the reference returns them (:87-109)."""
+        return [np.array(s, dtype=np.int64) for s in super().simulate(input_state, samples_number)]
+
 
 class LossyNetworksGeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
     def __init__(self, bs_permanent_calculator, seed: int = 0) -> None:
