@@ -46,6 +46,7 @@
 #ifndef TB_GRAD_SROW
 #define TB_GRAD_SROW 20
 #endif
+
 #ifndef TB_GRAD_ATTR
 #define TB_GRAD_ATTR __attribute__((amdgpu_waves_per_eu(2)))
 #endif
@@ -149,9 +150,23 @@ __device__ __forceinline__ void grad_loo(const double (&hr)[N], const double (&h
   }
 }
 
-template <int N, int S, int MODE>
+// Rows 0..B-1 in broadcast form (every static flip when B = U): no LDS read.
+template <int N, int B>
+struct GradRows {
+  Bcast<N> r[B > 0 ? B : 1];
+  double one;
+};
+
+template <int N, int S, int MODE, int B>
 __device__ __forceinline__ void grad_flip(double (&hr)[N], double (&hi)[N], const SRow<S, double> &sr,
-                                          const cpx<double> *M, int i, double sg) {
+                                          const GradRows<N, B> &gb, const cpx<double> *M, int i,
+                                          double sg) {
+  if (i < B) {  // compile-time at the static flip sites
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      if (u == i) flip_bcast<0, N, MODE>(hr, hi, gb.r[u], MODE == 0 ? sg : gb.one);
+    return;
+  }
   if (S > 0 && i == 0)
     flip_srow<N, S, MODE, double>(hr, hi, sr, M, sg);
   else
@@ -168,7 +183,10 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
   constexpr int G = kGradG;
   constexpr int NB = N - 1;
   constexpr int NO = N - G;
-  constexpr int S = (TB_GRAD_SROW < N) ? TB_GRAD_SROW : N;
+  // broadcast rows: all 3 static rows up to n = 24, rows 0-1 above (register budget); the
+  // 4-bit blocks of n <= 19 keep row 0 in SGPRs
+  constexpr int B = (U == 3) ? (N <= 24 ? 3 : 2) : 0;
+  constexpr int S = B > 0 ? 0 : ((TB_GRAD_SROW < N) ? TB_GRAD_SROW : N);
   __shared__ cpx<double> Ms[N * N];
   const int lane = threadIdx.x;
   const int p = lane & 15;
@@ -186,7 +204,14 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
   __syncthreads();
   const cpx<double> *M = Ms;
   SRow<S, double> sr;
-  load_srow(sr, M);
+  if constexpr (S > 0) load_srow(sr, M);
+  GradRows<N, B> gb;
+  if constexpr (B > 0) {
+#pragma unroll
+    for (int u = 0; u < B; ++u) load_bcast(gb.r[u], (const double *)(M + (size_t)u * N), lane);
+    gb.one = vgpr_one();
+    asm volatile("s_nop 4");  // VALU writes of the store before its first DPP read
+  }
   GradAcc acc = {0.0, 0.0, 0.0, 0.0};
   const int nblocks = 1 << (c - U);
   for (int it = 0; it < iters; ++it) {
@@ -235,14 +260,14 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
               Er[i][k] -= 2.0 * Tr[k];
               Ei[i][k] -= 2.0 * Ti[k];
             }
-            grad_flip<N, S, 1>(hr, hi, sr, M, i, 1.0);
+            grad_flip<N, S, 1, B>(hr, hi, sr, gb, M, i, 1.0);
           } else {
 #pragma unroll
             for (int k = 0; k < G; ++k) {
               Er[i][k] += 2.0 * Tr[k];
               Ei[i][k] += 2.0 * Ti[k];
             }
-            grad_flip<N, S, -1>(hr, hi, sr, M, i, -1.0);
+            grad_flip<N, S, -1, B>(hr, hi, sr, gb, M, i, -1.0);
           }
         } else {
           const bool t_next = (U < c) ? ((blk & 1) != 0) : q_odd;
@@ -252,7 +277,7 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
             Er[i][k] = fma(-2.0 * sg, Tr[k], Er[i][k]);
             Ei[i][k] = fma(-2.0 * sg, Ti[k], Ei[i][k]);
           }
-          grad_flip<N, S, 0>(hr, hi, sr, M, i, sg);
+          grad_flip<N, S, 0, B>(hr, hi, sr, gb, M, i, sg);
         }
         double pr[G], pi[G];
         grad_loo<N>(hr, hi, pr, pi);
