@@ -166,13 +166,20 @@ __device__ __forceinline__ void glynn_lane(P M, int C, uint64_t q0, uint64_t qst
             add = topbit_fixed ? false : q_odd;
           flip_row<N, R0, S, 0, F, P, B>(hr, hi, rr, sr, M, i, add ? F(1) : F(-1), &bs);
         }
-        cpx<F> p = TreeProdT<F, 0, N>::run(hr, hi);
-        if (j & 1) {
-          cre -= (double)p.re;
-          cim -= (double)p.im;
+        if constexpr (sizeof(F) == 8) {  // last multiply fused into the accumulation
+          if (j & 1)
+            tree_prod_acc<-1, 0, N>(cre, cim, (const double *)hr, (const double *)hi);
+          else
+            tree_prod_acc<1, 0, N>(cre, cim, (const double *)hr, (const double *)hi);
         } else {
-          cre += (double)p.re;
-          cim += (double)p.im;
+          cpx<F> p = TreeProdT<F, 0, N>::run(hr, hi);
+          if (j & 1) {
+            cre -= (double)p.re;
+            cim -= (double)p.im;
+          } else {
+            cre += (double)p.re;
+            cim += (double)p.im;
+          }
         }
         TB_STEP_FENCE();
       }
@@ -194,7 +201,9 @@ __device__ __forceinline__ void glynn_lane(P M, int C, uint64_t q0, uint64_t qst
       } else {
         flip_mem<N, 0, F>(hr, hi, M + (size_t)i * N, add ? F(1) : F(-1));
       }
-      {
+      if constexpr (sizeof(F) == 8) {
+        tree_prod_acc<1, 0, N>(cre, cim, (const double *)hr, (const double *)hi);
+      } else {
         cpx<F> p = TreeProdT<F, 0, N>::run(hr, hi);
         cre += (double)p.re;
         cim += (double)p.im;

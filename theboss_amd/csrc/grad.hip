@@ -157,6 +157,29 @@ struct GradRows {
   double one;
 };
 
+// T[k] += SIGN * P_k for the 4 leave-one-out products, each product's last multiply fused
+// into the accumulation (grad_loo followed by the adds, 6 instructions fewer).
+template <int SIGN, int N>
+__device__ __forceinline__ void grad_loo_acc(const double (&hr)[N], const double (&hi)[N],
+                                             double (&Tr)[kGradG], double (&Ti)[kGradG]) {
+  constexpr int NO = N - kGradG;
+  cd q = TreeProd<0, NO>::run(hr, hi);
+  cd suf[kGradG];
+  suf[kGradG - 2] = {hr[N - 1], hi[N - 1]};
+#pragma unroll
+  for (int k = kGradG - 3; k >= 0; --k) suf[k] = cmul({hr[NO + k + 1], hi[NO + k + 1]}, suf[k + 1]);
+#pragma unroll
+  for (int k = 0; k < kGradG; ++k) {
+    if (k == kGradG - 1) {
+      Tr[k] += SIGN * q.re;
+      Ti[k] += SIGN * q.im;
+    } else {
+      cmul_acc<SIGN>(Tr[k], Ti[k], q, suf[k]);
+    }
+    if (k + 1 < kGradG) q = cmul(q, {hr[NO + k], hi[NO + k]});
+  }
+}
+
 template <int N, int S, int MODE, int B>
 __device__ __forceinline__ void grad_flip(double (&hr)[N], double (&hi)[N], const SRow<S, double> &sr,
                                           const GradRows<N, B> &gb, const cpx<double> *M, int i,
@@ -183,9 +206,9 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
   constexpr int G = kGradG;
   constexpr int NB = N - 1;
   constexpr int NO = N - G;
-  // broadcast rows: all 3 static rows up to n = 24, rows 0-1 above (register budget); the
+  // broadcast rows (register budget): all 3 static rows up to n = 22, 2 up to 24, then 1; the
   // 4-bit blocks of n <= 19 keep row 0 in SGPRs
-  constexpr int B = (U == 3) ? (N <= 24 ? 3 : 2) : 0;
+  constexpr int B = (U == 3) ? (N <= 22 ? 3 : N <= 24 ? 2 : 1) : 0;
   constexpr int S = B > 0 ? 0 : ((TB_GRAD_SROW < N) ? TB_GRAD_SROW : N);
   __shared__ cpx<double> Ms[N * N];
   const int lane = threadIdx.x;
@@ -279,18 +302,10 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
           }
           grad_flip<N, S, 0, B>(hr, hi, sr, gb, M, i, sg);
         }
-        double pr[G], pi[G];
-        grad_loo<N>(hr, hi, pr, pi);
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-          if (jj & 1) {
-            Tr[k] -= pr[k];
-            Ti[k] -= pi[k];
-          } else {
-            Tr[k] += pr[k];
-            Ti[k] += pi[k];
-          }
-        }
+        if (jj & 1)
+          grad_loo_acc<-1, N>(hr, hi, Tr, Ti);
+        else
+          grad_loo_acc<1, N>(hr, hi, Tr, Ti);
         TB_STEP_FENCE();
       }
     };
@@ -305,15 +320,7 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
         grad_acc_row(acc, p, i, -2.0 * sg, zr, zi);
       }
       flip_mem<N, 0, double>(hr, hi, M + (size_t)i * N, sg);
-      {
-        double pr[G], pi[G];
-        grad_loo<N>(hr, hi, pr, pi);
-#pragma unroll
-        for (int k = 0; k < G; ++k) {
-          Tr[k] += pr[k];
-          Ti[k] += pi[k];
-        }
-      }
+      grad_loo_acc<1, N>(hr, hi, Tr, Ti);
       TB_STEP_FENCE();
       static_steps(blk);
     }

@@ -227,14 +227,10 @@ __device__ __forceinline__ void sweep_chunk(const cd *Vs, const Col0<K - 1, R0> 
           }
         }
       }
-      const cd p = sweep_prod<R>(hr, hi);
-      if (j & 1) {
-        Tr -= p.re;
-        Ti -= p.im;
-      } else {
-        Tr += p.re;
-        Ti += p.im;
-      }
+      if (j & 1)
+        tree_prod_acc<-1, 0, R>(Tr, Ti, hr, hi);
+      else
+        tree_prod_acc<1, 0, R>(Tr, Ti, hr, hi);
       TB_STEP_FENCE();
     }
   };
@@ -262,11 +258,7 @@ __device__ __forceinline__ void sweep_chunk(const cd *Vs, const Col0<K - 1, R0> 
     } else {
       sweep_flip<R, 0, kPre>(hr, hi, Vs + (i + 1) * R, sg, pre);
     }
-    {
-      const cd p = sweep_prod<R>(hr, hi);
-      Tr += p.re;
-      Ti += p.im;
-    }
+    tree_prod_acc<1, 0, R>(Tr, Ti, hr, hi);
     TB_STEP_FENCE();
     static_steps(blk);
   }

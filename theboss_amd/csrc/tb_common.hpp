@@ -48,6 +48,25 @@ struct TreeProd<LO, LO> {
   static __device__ __forceinline__ cd run(const double *, const double *) { return {1.0, 0.0}; }
 };
 
+// acc += SIGN * a * b with the multiply fused into the accumulation: 4 FMAs instead of a
+// complex multiply (4 instructions) followed by 2 adds.
+template <int SIGN>
+__device__ __forceinline__ void cmul_acc(double &ar, double &ai, cd a, cd b) {
+  constexpr double s = SIGN;
+  ar = fma(s * a.re, b.re, ar);
+  ar = fma(-s * a.im, b.im, ar);
+  ai = fma(s * a.re, b.im, ai);
+  ai = fma(s * a.im, b.re, ai);
+}
+
+// acc += SIGN * prod(re[LO..HI) + i im[LO..HI)), the tree's last multiply fused.
+template <int SIGN, int LO, int HI>
+__device__ __forceinline__ void tree_prod_acc(double &ar, double &ai, const double *re,
+                                              const double *im) {
+  constexpr int MID = (LO + HI) / 2;
+  cmul_acc<SIGN>(ar, ai, TreeProd<LO, MID>::run(re, im), TreeProd<MID, HI>::run(re, im));
+}
+
 // Butterfly sum over the 64 lanes of a wave; every lane receives the total.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
