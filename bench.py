@@ -69,7 +69,7 @@ def main() -> int:
 
     from oracle import oracle as orc  # checker + CPU baseline only
     from theboss_amd import _device as D
-    from theboss_amd import ops
+    from theboss_amd import autotune, ops
     from theboss_amd.permanent_calculators import GPURyserPermanentCalculator
     from theboss_amd.simulation_strategies import GeneralizedCliffordsBSimulationStrategy
 
@@ -99,8 +99,6 @@ def main() -> int:
     # ---- autotuner: chunk size of each step's sweep at this batch (one-time set-up)
     tuned = None
     if not args.no_autotune:
-        from theboss_amd import autotune
-
         res, secs = autotune.timed(autotune.tune_sampler, M, N, SAMPLES)
         tuned = {"seconds": secs,
                  "chunk_bits": {k: r["chunk_bits"] for k, r in res.items()},
@@ -201,7 +199,8 @@ def main() -> int:
         "fp64_tflops_sampler": batched * sample_flops(N, M) / 1e12,
         "roofline": {
             "kernel": "sweep_kernel<20,3,3> (C&C minors sweep, k=20, 4096 items, chunk 2^%d)"
-                      % (tuned["chunk_bits"].get(N, 11) if tuned else 11),
+                      % (tuned["chunk_bits"].get(N, autotune.sweep_builtin(N)) if tuned
+                         else autotune.sweep_builtin(N)),
             "flops_per_launch": SAMPLES * sweep_flops(k),
             "bound": "valu",
             "achieved": achieved,

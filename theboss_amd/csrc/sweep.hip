@@ -387,15 +387,19 @@ struct SweepPlan {
   int c, wpi, iters;
 };
 
+#ifndef TB_SWEEP_BUILTIN_MAX_C
+#define TB_SWEEP_BUILTIN_MAX_C 11
+#endif
 constexpr int64_t kSweepMaxBlocks = 8192;  // 4 one-wave blocks per SIMD of 256 CUs x 4 SIMDs x 2
 
-// Default chunk: 2^min(11, k-6) ranks per lane (>= 2^U), so one wave holds 64 chunks and a
-// lane walks at most 2048 Gray steps between re-seeds.
+// Default chunk: 2^min(11, k-7) ranks per lane (>= 2^U), so an item has at least 64 chunks
+// (one wave) and a lane walks at most 2048 Gray steps between re-seeds. The tuner may pick
+// longer walks (fewer seedings and reductions) where the batch still fills the GPU.
 inline SweepPlan sweep_plan(int k, int c_override) {
   SweepPlan p{};
   const int nb = k - 1;                      // Gray bits (particle 0 is fixed)
   const int u = std::min(tb::kSweepU, nb);   // static bits of the kernel for this k
-  int c = c_override > 0 ? c_override : std::min(11, nb - 6);
+  int c = c_override > 0 ? c_override : std::min(TB_SWEEP_BUILTIN_MAX_C, nb - 6);
   c = std::max(c, u);
   c = std::min(c, std::min(nb, tb::kSweepMaxC));
   p.c = c;
