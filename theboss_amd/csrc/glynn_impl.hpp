@@ -310,8 +310,11 @@ namespace tbgl {
 #ifndef TB_GLYNN_F64_R0_MAX_N
 #define TB_GLYNN_F64_R0_MAX_N 24
 #endif
+// Up to n = 40 the broadcast rows stay in VGPRs; above, the compiler parks them in AGPRs
+// and copies them back right before the DPP reads (a VALU write the DPP read must not
+// follow within 2 wait states), so larger n keep row 0 in SGPRs instead.
 #ifndef TB_GLYNN_F64_BCAST_MAX_N
-#define TB_GLYNN_F64_BCAST_MAX_N 48
+#define TB_GLYNN_F64_BCAST_MAX_N 40
 #endif
 template <int N, typename F>
 constexpr int glynn_reg_rows() {
