@@ -81,13 +81,14 @@ def test_glynn_fp64_closed_forms(T, ops, oracle, n):
         assert abs(g - e) <= REL * abs(e), (n, g, e)
 
 
-def test_glynn_fp64_rank_one_at_n40(T, ops, oracle):
-    """Near the largest supported size (2^39 Gray steps): perm(u v^T) = n! prod u prod v
-    with unit-modulus entries, whose Glynn terms stay far below the result."""
+@pytest.mark.parametrize("n", [40, 41])
+def test_glynn_fp64_rank_one_large(T, ops, oracle, n):
+    """Near the largest supported size (2^39 and 2^40 Gray steps; n = 40 is the largest
+    with broadcast rows, n = 41 keeps row 0 in SGPRs): perm(u v^T) = n! prod u prod v with
+    unit-modulus entries, whose Glynn terms stay far below the result."""
     import math
 
-    n = 40
-    rng = np.random.default_rng(40)
+    rng = np.random.default_rng(n)
     u, v = np.exp(2j * np.pi * rng.random(n)), np.exp(2j * np.pi * rng.random(n))
     got = ops.glynn_batch(dev_c(T, np.outer(u, v)[None])).cpu().numpy()[0]
     expect = math.factorial(n) * np.prod(u) * np.prod(v)
