@@ -235,11 +235,47 @@ def test_submatrices_collision_inputs_vs_oracle(T, oracle):
         assert np.max(np.abs(got - ref)) <= 1e-12, (s, t)
 
 
+def test_sweep_every_k_laplace_vs_glynn(T, ops, oracle):
+    """Every sweep instantiation k = 2..31 (broadcast columns; the lane offsets, group
+    counts and static bits differ per k) and the SGPR-row ones up to k = 34: the Laplace
+    identity against the Glynn kernel, and against the oracle up to k = 20."""
+    for k in range(2, 35):
+        A = oracle.haar_unitary(2 * k, 950 + k)[:k, :k]
+        rows = sorted({0, k // 2, k - 1})
+        Vts = np.stack([np.delete(A, o, axis=0).T for o in rows])
+        minors = ops.sweep_minors_batch(dev_c(T, Vts), k).cpu().numpy()
+        perm = ops.glynn_batch(dev_c(T, A[None])).cpu().numpy()[0]
+        if k <= 20:
+            ref = oracle.perm_glynn_matrix(A, 1)
+            assert abs(perm - ref) <= REL * abs(ref), k
+        for row, o in enumerate(rows):
+            lap = np.sum(A[o, :] * minors[row])
+            assert abs(lap - perm) <= REL * abs(perm), (k, o, lap, perm)
+
+
+def test_glynn_every_broadcast_n(T, ops, oracle):
+    """Every FP64 wide-kernel instantiation with broadcast rows: n = 11..26 against the
+    oracle, n = 27..39 by the rank-one closed form (n = 40 has its own test)."""
+    import math
+
+    for n in range(11, 27):
+        As = np.stack([oracle.haar_unitary(2 * n, 4000 + 10 * n + b)[:n, :n] for b in range(2)])
+        got = ops.glynn_batch(dev_c(T, As)).cpu().numpy()
+        ref = oracle.perm_glynn_batch(As, 1)
+        assert np.max(np.abs(got - ref) / np.abs(ref)) <= REL, n
+    for n in range(27, 40):
+        rng = np.random.default_rng(n)
+        u, v = np.exp(2j * np.pi * rng.random(n)), np.exp(2j * np.pi * rng.random(n))
+        got = ops.glynn_batch(dev_c(T, np.outer(u, v)[None])).cpu().numpy()[0]
+        expect = math.factorial(n) * np.prod(u) * np.prod(v)
+        assert abs(got - expect) <= REL * abs(expect), n
+
+
 @pytest.mark.parametrize("k", [22, 26, 32])
 def test_sweep_large_k_laplace_vs_glynn(T, ops, oracle, k):
-    """k > 21 takes particle 1's column from SGPRs. For every output row o of a random
-    k x k matrix A: sum_nu A[o][nu] minor_nu(A without row o) = perm(A), with perm(A) from
-    the Glynn kernel (and from the oracle at k = 22)."""
+    """Broadcast columns up to k = 31, particle 1's first values in SGPRs above. For
+    every output row o of a random k x k matrix A: sum_nu A[o][nu] minor_nu(A without
+    row o) = perm(A), with perm(A) from the Glynn kernel (and from the oracle at k = 22)."""
     A = oracle.haar_unitary(2 * k, 900 + k)[:k, :k]
     Vts = np.stack([np.delete(A, o, axis=0).T for o in (0, k // 2, k - 1)])  # particle-major
     minors = ops.sweep_minors_batch(dev_c(T, Vts), k).cpu().numpy()
@@ -305,11 +341,11 @@ def test_grad_single_sweep_vs_oracle_minors(T, ops, oracle, n, B):
         assert abs(perm[b] - p) <= REL * abs(p), (n, b)
 
 
-@pytest.mark.parametrize("n", [23, 26])
+@pytest.mark.parametrize("n", [16, 18, 23, 24, 25, 26])
 def test_grad_single_sweep_laplace_large(T, ops, oracle, n):
-    """n = 23, 26 (too slow for all n^2 oracle minors): Laplace expansion along every row
-    and every column equals the Glynn kernel's permanent, and three minors against the
-    oracle."""
+    """Sizes outside the all-minors oracle test (n >= 23 is too slow for all n^2 oracle
+    minors): Laplace expansion along every row and every column equals the Glynn
+    kernel's permanent, and three minors against the oracle."""
     As = np.stack([oracle.haar_unitary(2 * n, 900 + n + b)[:n, :n] for b in range(2)])
     Ad = dev_c(T, As)
     perm, G = ops.perm_grad_batch(Ad)
