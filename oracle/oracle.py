@@ -20,6 +20,7 @@ How each routine is pinned (tests/test_oracle.py):
                           J_n gives n!, derangements); perm(cA)=c^n perm(A); block
                           diagonal factorises; transpose and row/column permutation
                           invariance.
+* ryser_py (timing)    -- equals the C ryser on random multiplicity patterns.
 * ryser / chin-huh     -- equal classic on the nine occupation patterns of the
                           reference's tests/test_bs_permanent_calculators.py:88-122
                           and on random multiplicity patterns.
@@ -164,6 +165,46 @@ def perm_glynn_batch(A, mode: int = 1, threads: int = 0) -> np.ndarray:
 def perm_ryser(U, s, t) -> complex:
     U, s, t = _cm(U), _ci(s), _ci(t)
     return _c(lib().orc_perm_guan(_p(U), U.shape[0], _p(s), _p(t), 0))
+
+
+def perm_ryser_py(U, s, t) -> complex:
+    """Pure-Python restatement of the reference's Ryser walk, kept for timing the
+    reference's speed class (bench's GCC-A CPU line): the Guan (reflected mixed-radix
+    Gray) walk over r <= s across all m digits, zero-limit digits included
+    (bs_permanent_calculator_base.py:123-209), the running binomial product Pi C(s_i, r_i)
+    by ratios (:151-164), and sums_j += dr * U[j][i] with the addend
+    (-1)^(|s| + |r|) Pi C(s, r) Pi_j sums_j^t_j (ryser_permanent_calculator.py:45-64).
+    The digit step follows Knuth's reflected mixed-radix Gray code (the lowest digit that
+    can move in its direction moves; the exhausted ones below it turn around)."""
+    Ul = np.asarray(U, dtype=np.complex128).tolist()
+    s = [int(x) for x in s]
+    t = [int(x) for x in t]
+    m = len(s)
+    rows = [j for j in range(m) if t[j] > 0]
+    r = [0] * m
+    d = [1] * m
+    sums = [0j] * m
+    binom = 1.0
+    sign = 1
+    total = 0j
+    while True:
+        i = 0
+        while i < m and not (0 <= r[i] + d[i] <= s[i]):
+            d[i] = -d[i]
+            i += 1
+        if i == m:
+            break
+        old = r[i]
+        r[i] = old + d[i]
+        binom = binom * (s[i] - old) / (old + 1) if d[i] > 0 else binom * old / (s[i] - old + 1)
+        sign = -sign
+        for j in rows:
+            sums[j] += d[i] * Ul[j][i]
+        prod = 1 + 0j
+        for j in rows:
+            prod *= sums[j] ** t[j]
+        total += sign * binom * prod
+    return complex((-1) ** sum(s) * total)
 
 
 def perm_chin_huh(U, s, t) -> complex:

@@ -128,6 +128,20 @@ def test_guan_calculators_random_multiplicities(oracle):
         expect = oracle.perm_classic(U, s, t)
         assert abs(oracle.perm_ryser(U, s, t) - expect) < 1e-12
         assert abs(oracle.perm_chin_huh(U, s, t) - expect) < 1e-12
+        assert abs(oracle.perm_ryser_py(U, s, t) - expect) < 1e-12
+
+
+def test_python_ryser_walk_on_gcca_shapes(oracle):
+    """The pure-Python Ryser walk (timing baseline) on GCC-A shapes: a 0/1 substate of the
+    input against a bunched output, over all m digits, equals the C walk."""
+    rng = np.random.default_rng(6)
+    m = 12
+    U = oracle.haar_unitary(m, 300)
+    for k in (1, 3, 6):
+        s = np.zeros(m, int)
+        s[rng.choice(6, k, replace=False)] = 1
+        t = np.bincount(rng.integers(0, m, k), minlength=m)
+        assert abs(oracle.perm_ryser_py(U, s, t) - oracle.perm_ryser(U, s, t)) < 1e-12
 
 
 # Cases of the reference's tests/test_bs_submatrices_permanent_calculators.py:41-73.

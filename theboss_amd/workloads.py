@@ -4,6 +4,7 @@ GCC-A, uniform-loss GCC-B, interferometer fitting). Each returns one JSON-able r
 `checker` is the CPU oracle module passed in by the caller (bench.py); it is used only
 to produce seeded inputs and reference values, never for the measured computation.
 """
+import collections
 import json
 import math
 import os
@@ -133,8 +134,30 @@ def gcca(checker):
     ones = [1] * n + [0] * (m - n)
     r = GPURyserPermanentCalculator(U, ones, ones).compute_permanent()
     g = GPUGlynnGrayPermanentCalculator(U, ones, ones).compute_permanent()
+    # CPU line of the reference's speed class: the layer pmf with the checker's pure-Python
+    # Ryser walk (the reference's loop structure), timed on a few chains of one layer per
+    # particle number and scaled by the number of layers of each particle number this run
+    # computed (the memo of the reference computes the same layers).
+    counts = collections.Counter(sum(key) + 1 for key in strat.pmfs)
+    per_k = collections.defaultdict(list)
+    rng = np.random.default_rng(1)
+    spent, chains = 0.0, 0
+    while spent < 6.0 and chains < 4:
+        prefix = [0] * m
+        for k in range(1, n + 1):
+            t1 = time.perf_counter()
+            pmf = checker.gcca_layer_pmf(U, inp, prefix, perm=checker.perm_ryser_py)
+            per_k[k].append(time.perf_counter() - t1)
+            prefix[int(rng.choice(m, p=pmf / pmf.sum()))] += 1
+        spent += sum(per_k[k][-1] for k in range(1, n + 1))
+        chains += 1
+    est = sum(float(np.mean(per_k[k])) * c for k, c in counts.items())
+    cpu = {"value": S / est, "unit": "samples/s", "cores": 1, "kind": "port",
+           "sample": f"{chains} chains of the {n} layer pmfs with a pure-Python Ryser walk, "
+                     f"{spent:.1f} s, scaled by this run's {len(strat.pmfs)} layers"}
     return {"workload": "GCC-A m=20 n=10, 200 samples (Ryser, lock-step layers)",
             "samples_per_s": S / dt, "seconds": dt, "layers_computed": len(strat.pmfs),
+            "cpu_python_baseline": cpu, "gpu_vs_cpu_python": (S / dt) / cpu["value"],
             "ryser_vs_glynn_10x10_rel": abs(r - g) / abs(g),
             "all_samples_have_n_photons": all(sum(s_) == n for s_ in samples)}
 
