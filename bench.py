@@ -151,7 +151,7 @@ def main() -> int:
             traffic = json.load(f).get("bytes_per_launch")
 
     # ---- CPU baseline: the oracle's GCC-B sampler on the host cores, bounded sample
-    cpu = None
+    cpu = cpu_py = None
     if not args.no_cpu_baseline:
         cores = max(1, min(16, os.cpu_count() or 1))
         rng = np.random.default_rng(SEED)
@@ -166,6 +166,26 @@ def main() -> int:
         cpu = {"value": done / spent, "unit": "samples/s", "cores": cores, "kind": "port",
                "sample": f"{done} GCC-B samples at n={N}, m={M} (same Haar matrix) by the "
                          f"oracle's C restatement of the reference sampler, OpenMP over samples"}
+        # The reference's own speed class: its C&C walk restated in pure Python on this m.
+        # Its cost per Gray step at k = 12..13 (≈ constant there, dominated by the pass over
+        # all m minors) times the 2^k steps of every k of one sample estimates its time per
+        # sample.
+        per_step = []
+        for kk in (12, 13):
+            s_in = np.zeros(M, dtype=np.int64)
+            s_in[:kk] = 1
+            t_out = np.zeros(M, dtype=np.int64)
+            t_out[np.random.default_rng(kk).choice(M, kk - 1, replace=False)] = 1
+            t0 = time.perf_counter()
+            orc.cc_submatrices_py(U, s_in, t_out)
+            per_step.append((time.perf_counter() - t0) / 2 ** kk)
+        step_s = min(per_step)
+        sample_s = step_s * sum(2 ** k for k in range(2, N + 1))
+        cpu_py = {"value": 1.0 / sample_s, "unit": "samples/s", "cores": 1, "kind": "port",
+                  "sample": f"pure-Python restatement of the reference's C&C Ryser walk on "
+                            f"m={M}: cost per Gray step at k = 12..13 ({1e6 * step_s:.1f} us), "
+                            f"times the 2^k steps of k = 2..{N} of one sample; the pmf and draw "
+                            f"work is not counted"}
 
     # ---- the metric's second part: permanent + gradient (64 x n=20) and single large
     # permanents (16 x n=32) as % of the FP64 vector peak (full records under "workloads")
@@ -211,6 +231,7 @@ def main() -> int:
             "launch_ms": t_launch * 1e3,
         },
         "cpu_baseline": cpu,
+        "cpu_python_estimate": cpu_py,
         "autotune": tuned,
         "perm_grad_frac_fp64_peak": wl_grad["frac_fp64_peak"],
         "glynn_n32_frac_fp64_peak": wl_glynn["results"][0]["fp64"]["frac_fp64_peak"],

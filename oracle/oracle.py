@@ -20,7 +20,8 @@ How each routine is pinned (tests/test_oracle.py):
                           J_n gives n!, derangements); perm(cA)=c^n perm(A); block
                           diagonal factorises; transpose and row/column permutation
                           invariance.
-* ryser_py (timing)    -- equals the C ryser on random multiplicity patterns.
+* ryser_py, cc_submatrices_py (timing restatements in pure Python)
+                       -- equal the C ryser / cc submatrices on random patterns.
 * ryser / chin-huh     -- equal classic on the nine occupation patterns of the
                           reference's tests/test_bs_permanent_calculators.py:88-122
                           and on random multiplicity patterns.
@@ -205,6 +206,49 @@ def perm_ryser_py(U, s, t) -> complex:
             prod *= sums[j] ** t[j]
         total += sign * binom * prod
     return complex((-1) ** sum(s) * total)
+
+
+def cc_submatrices_py(U, s, t) -> List[complex]:
+    """Pure-Python restatement of the reference's C&C Ryser submatrices walk, kept for
+    timing the reference's speed class on the GCC-B workload: the Guan digit scan over all
+    m modes (bs_submatrices_permanent_calculator_base.py:99-148, 160-175), the sums over the
+    occupied outputs with their powers (bs_cc_ryser_submatrices_permanent_calculator.py:80-94)
+    and the per-step pass over all m minors (:96-119)."""
+    Ul = np.asarray(U, dtype=np.complex128).tolist()
+    s = [int(x) for x in s]
+    t = [int(x) for x in t]
+    m = len(s)
+    if sum(s) == 1:
+        return [complex(v) for v in s]
+    perms = [0j] * m
+    sums = {j: 0j for j in range(m) if t[j] != 0}
+    mult = (-1) ** sum(t)
+    binom = 1.0
+    r = [0] * m
+    u = [1] * m
+    while r[-1] <= s[-1]:
+        i = 0
+        val = r[0] + u[0]
+        while val > s[i] or val < 0:
+            u[i] = -u[i]
+            i += 1
+            if i == m:
+                return perms
+            val = r[i] + u[i]
+        last = r[i]
+        r[i] = val
+        mult = -mult
+        binom *= (s[i] - last) / r[i] if r[i] > last else last / (s[i] - r[i])
+        prod = 1 + 0j
+        for j in sums:
+            sums[j] += (r[i] - last) * Ul[j][i]
+            prod *= sums[j] ** t[j]
+        prod *= mult
+        for q in range(m):
+            if s[q] == 0 or r[q] == s[q]:
+                continue
+            perms[q] += prod * (binom / (s[q] / (s[q] - r[q])))
+    return perms
 
 
 def perm_chin_huh(U, s, t) -> complex:

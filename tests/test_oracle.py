@@ -131,6 +131,20 @@ def test_guan_calculators_random_multiplicities(oracle):
         assert abs(oracle.perm_ryser_py(U, s, t) - expect) < 1e-12
 
 
+def test_python_cc_walk_equals_c_walk(oracle):
+    """The pure-Python C&C walk (timing baseline) equals the C restatement, collision
+    inputs and zero-limit digits included."""
+    rng = np.random.default_rng(8)
+    for trial in range(12):
+        m = int(rng.integers(3, 8))
+        k = int(rng.integers(2, 7))
+        U = oracle.haar_unitary(m, 400 + trial)
+        s = np.bincount(rng.integers(0, m, k), minlength=m)
+        t = np.bincount(rng.integers(0, m, k - 1), minlength=m)
+        got = np.array(oracle.cc_submatrices_py(U, s, t))
+        assert np.max(np.abs(got - oracle.cc_submatrices(U, s, t))) < 1e-12, (s, t)
+
+
 def test_python_ryser_walk_on_gcca_shapes(oracle):
     """The pure-Python Ryser walk (timing baseline) on GCC-A shapes: a 0/1 substate of the
     input against a bunched output, over all m digits, equals the C walk."""
