@@ -31,9 +31,9 @@
 
 namespace tbs {
 size_t sweep_partial_bytes(int k, int items, int c_override);
-int32_t run_sweep(int k, const tb::cd *Vt, int ldv, int64_t item_stride, const int *item_map,
-                  const int *item_count, int items, int c_override, tb::cd *partial,
-                  tb::cd *minors, hipStream_t s);
+int32_t run_sweep_partial(int k, const tb::cd *Vt, int ldv, int64_t item_stride,
+                          const int *item_map, const int *item_count, int items, int c_override,
+                          tb::cd *partial, int *wpi, hipStream_t s);
 }  // namespace tbs
 
 namespace tb {
@@ -97,85 +97,121 @@ __device__ __forceinline__ double binom_d(int n, int k) {
   return rint(r);
 }
 
-// Per sample: particle order, number of steps, zeroed output row.
-__global__ void gccb_init_kernel(const GccbParams *__restrict__ prm, int m, int n, int S,
-                                 int *__restrict__ order, int *__restrict__ steps) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S) return;
-  const uint64_t seed = prm->seed;
-  const double eta = prm->eta;
-  const int *forced_order = prm->forced_order;
-  const int *forced_steps = prm->forced_steps;
+// Per sample: particle order, number of steps, zeroed output row. One wave per 64
+// samples: each lane shuffles its order in LDS (column lane of ord_lds), and the wave
+// writes the orders and zeroes the output rows of its samples with coalesced stores.
+constexpr int kInitBlock = 64;
+__global__ __launch_bounds__(kInitBlock) void gccb_init_kernel(
+    const GccbParams *__restrict__ prm, int m, int n, int S, int *__restrict__ order,
+    int *__restrict__ steps) {
+  __shared__ int ord_lds[TB_SWEEP_MAX_K * kInitBlock];  // ord_lds[i * kInitBlock + lane]
+  const int lane = threadIdx.x;
+  const int s0 = blockIdx.x * kInitBlock;
+  const int rows = min(kInitBlock, S - s0);
+  const int s = s0 + lane;
   int *out_samples = prm->out_samples;
-  int *ord = order + (size_t)s * n;
-  // Per-sample particle lists hold the sample's forced_steps[s] particles first: only that
-  // prefix is shuffled, so the padding beyond it never enters a step.
-  const int span = (prm->per_sample && forced_steps) ? min(max(forced_steps[s], 0), n) : n;
-  if (forced_order) {
-    for (int i = 0; i < n; ++i) ord[i] = forced_order[(size_t)s * n + i];
-  } else {
-    for (int i = 0; i < n; ++i) ord[i] = i;
-    for (int i = span - 1; i >= 1; --i) {  // Fisher-Yates
-      const double u = philox_uniform(seed, (uint32_t)s, (uint32_t)i, kPurposeOrder);
-      int j = (int)(u * (double)(i + 1));
-      j = j > i ? i : j;
-      const int t = ord[i];
-      ord[i] = ord[j];
-      ord[j] = t;
-    }
-  }
-  int l;
-  if (forced_steps) {
-    l = forced_steps[s];
-  } else if (eta >= 1.0) {
-    l = n;
-  } else {
-    // generalized_cliffords_b_uniform_losses_simulation_strategy.py:56-85
-    const double u = philox_uniform(seed, (uint32_t)s, 0u, kPurposeLoss);
-    double acc = 0.0;
-    l = n;
-    for (int k = 0; k <= n; ++k) {
-      acc += binom_d(n, k) * pow(eta, (double)k) * pow(1.0 - eta, (double)(n - k));
-      if (acc > u) {
-        l = k;
-        break;
+  for (int i = lane; i < rows * m; i += kInitBlock) out_samples[(size_t)s0 * m + i] = 0;
+  if (s < S) {
+    const uint64_t seed = prm->seed;
+    const double eta = prm->eta;
+    const int *forced_order = prm->forced_order;
+    const int *forced_steps = prm->forced_steps;
+    int *ord = ord_lds + lane;
+    // Per-sample particle lists hold the sample's forced_steps[s] particles first: only
+    // that prefix is shuffled, so the padding beyond it never enters a step.
+    const int span = (prm->per_sample && forced_steps) ? min(max(forced_steps[s], 0), n) : n;
+    if (forced_order) {
+      for (int i = 0; i < n; ++i) ord[i * kInitBlock] = forced_order[(size_t)s * n + i];
+    } else {
+      for (int i = 0; i < n; ++i) ord[i * kInitBlock] = i;
+      for (int i = span - 1; i >= 1; --i) {  // Fisher-Yates
+        const double u = philox_uniform(seed, (uint32_t)s, (uint32_t)i, kPurposeOrder);
+        int j = (int)(u * (double)(i + 1));
+        j = j > i ? i : j;
+        const int t = ord[i * kInitBlock];
+        ord[i * kInitBlock] = ord[j * kInitBlock];
+        ord[j * kInitBlock] = t;
       }
     }
+    int l;
+    if (forced_steps) {
+      l = forced_steps[s];
+    } else if (eta >= 1.0) {
+      l = n;
+    } else {
+      // generalized_cliffords_b_uniform_losses_simulation_strategy.py:56-85
+      const double u = philox_uniform(seed, (uint32_t)s, 0u, kPurposeLoss);
+      double acc = 0.0;
+      l = n;
+      for (int k = 0; k <= n; ++k) {
+        acc += binom_d(n, k) * pow(eta, (double)k) * pow(1.0 - eta, (double)(n - k));
+        if (acc > u) {
+          l = k;
+          break;
+        }
+      }
+    }
+    l = l < 0 ? 0 : (l > n ? n : l);
+    steps[s] = l;
   }
-  l = l < 0 ? 0 : (l > n ? n : l);
-  steps[s] = l;
-  for (int mm = 0; mm < m; ++mm) out_samples[(size_t)s * m + mm] = 0;
+  __syncthreads();
+  // order[(s0 + r) * n + i] = ord_lds[i * kInitBlock + r]: the block's rows are contiguous.
+  for (int e = lane; e < rows * n; e += kInitBlock) {
+    const int r = e / n, i = e - r * n;
+    order[(size_t)s0 * n + e] = ord_lds[i * kInitBlock + r];
+  }
 }
 
-// Single workgroup: slots ordered by steps (descending, stable) and count[k] = number of
-// samples with steps >= k, k = 0..n.
+// Single workgroup of 16 waves: slots ordered by steps (descending, stable in the sample
+// index) and count[v] = number of samples with steps >= v, v = 0..n. Wave w owns the
+// contiguous samples [w * seg, (w + 1) * seg): it counts each value there, the offsets
+// follow (larger values first, then lower waves), and the wave scatters its samples in
+// index order, ranking the lanes of a 64-sample slice that share a value by ballot.
+// steps[] holds values in 0..n (the init kernel clamps them).
 __global__ __launch_bounds__(1024) void gccb_order_kernel(const int *__restrict__ steps, int S,
                                                           int n, int *__restrict__ slot_map,
                                                           int *__restrict__ count) {
-  __shared__ int scan[1024];
-  __shared__ int base;
-  const int tid = threadIdx.x;
-  if (tid == 0) base = 0;
+  constexpr int W = 16, V = TB_SWEEP_MAX_K + 1;
+  __shared__ int cnt[W][V];
+  __shared__ int tot[V];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  for (int i = tid; i < W * V; i += 1024) cnt[i / V][i % V] = 0;
   __syncthreads();
-  for (int v = n; v >= 0; --v) {
-    for (int start = 0; start < S; start += 1024) {
-      const int s = start + tid;
-      const int flag = (s < S && steps[s] == v) ? 1 : 0;
-      scan[tid] = flag;
-      __syncthreads();
-      for (int off = 1; off < 1024; off <<= 1) {
-        const int add = tid >= off ? scan[tid - off] : 0;
-        __syncthreads();
-        scan[tid] += add;
-        __syncthreads();
-      }
-      if (flag) slot_map[base + scan[tid] - 1] = s;
-      __syncthreads();
-      if (tid == 0) base += scan[1023];
-      __syncthreads();
+  const int seg = (S + W - 1) / W;
+  const int lo = min(S, w * seg), hi = min(S, lo + seg);
+  for (int s = lo + lane; s < hi; s += 64) atomicAdd(&cnt[w][steps[s]], 1);
+  __syncthreads();
+  if (tid <= n) {
+    int t = 0;
+    for (int ww = 0; ww < W; ++ww) t += cnt[ww][tid];
+    tot[tid] = t;
+  }
+  __syncthreads();
+  if (tid <= n) {
+    int base = 0;
+    for (int v = n; v > tid; --v) base += tot[v];
+    for (int ww = 0; ww < W; ++ww) {
+      const int c = cnt[ww][tid];
+      cnt[ww][tid] = base;
+      base += c;
     }
-    if (tid == 0) count[v] = base;  // samples with steps >= v
-    __syncthreads();
+    count[tid] = base;
+  }
+  __syncthreads();
+  for (int s0 = lo; s0 < hi; s0 += 64) {
+    const int s = s0 + lane;
+    const bool in = s < hi;
+    const int v = in ? steps[s] : -1;
+    unsigned long long todo = __ballot(in);
+    while (todo) {
+      const int leader = __ffsll((long long)todo) - 1;
+      const int lv = __shfl(v, leader, 64);
+      const unsigned long long same = __ballot(in && v == lv);
+      const int at = cnt[w][lv];
+      if (in && v == lv) slot_map[at + __popcll(same & ((1ull << lane) - 1ull))] = s;
+      if (lane == leader) cnt[w][lv] = at + __popcll(same);
+      todo &= ~same;
+    }
   }
 }
 
@@ -192,12 +228,14 @@ __global__ void gccb_uin_kernel(const GccbParams *__restrict__ prm, int m, int n
   UinT[idx] = U[(size_t)mm * prm->ldu + prm->particles[(size_t)s * n + nu]];
 }
 
-// One wave per active slot at step k: Laplace over m modes, inverse-CDF draw, output and
-// column bookkeeping for the next step.
+// One wave per active slot at step k: the sweep's wpi partial sums of the k minors (added
+// in w order), Laplace over m modes, inverse-CDF draw, output and column bookkeeping for
+// the next step.
 __global__ __launch_bounds__(64) void gccb_step_kernel(
     const GccbParams *__restrict__ prm, int k, int m, int n, int S, const int *__restrict__ count,
-    const int *__restrict__ slot_map, const cd *__restrict__ minors, const cd *__restrict__ UinT,
-    const int *__restrict__ order, int *__restrict__ outs, cd *__restrict__ Vt) {
+    const int *__restrict__ slot_map, const cd *__restrict__ partial, int wpi,
+    const cd *__restrict__ UinT, const int *__restrict__ order, int *__restrict__ outs,
+    cd *__restrict__ Vt) {
   extern __shared__ char smem[];
   const double *forced_uniforms = prm->forced_uniforms;
   int *out_samples = prm->out_samples;
@@ -212,7 +250,17 @@ __global__ __launch_bounds__(64) void gccb_step_kernel(
   const int *ord = order + (size_t)s * n;
   if (prm->per_sample) UinT += (size_t)s * n * m;
   if (lane < k) {
-    mi[lane] = (k == 1) ? cd{1.0, 0.0} : minors[(size_t)slot * k + lane];
+    cd v{1.0, 0.0};
+    if (k > 1) {
+      double re = 0.0, im = 0.0;
+      const cd *p = partial + (size_t)slot * wpi * k + lane;
+      for (int w = 0; w < wpi; ++w) {
+        re += p[(size_t)w * k].re;
+        im += p[(size_t)w * k].im;
+      }
+      v = {re, im};
+    }
+    mi[lane] = v;
     rows[lane] = ord[lane];
   }
   __syncthreads();
@@ -298,7 +346,7 @@ namespace {
 inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
-  size_t params, uint, order, steps, slot_map, count, outs, vt, minors, partial, total;
+  size_t params, uint, order, steps, slot_map, count, outs, vt, partial, total;
 };
 
 Layout layout(const tb_gccb_config &c) {
@@ -318,7 +366,6 @@ Layout layout(const tb_gccb_config &c) {
   L.count = take((n + 2) * sizeof(int));
   L.outs = take(S * n * sizeof(int));
   L.vt = take(S * n * n * sizeof(tb::cd));
-  L.minors = take(S * n * sizeof(tb::cd));
   size_t part = 0;
   for (int k = 2; k <= c.n; ++k) part = std::max(part, tbs::sweep_partial_bytes(k, c.samples, c.chunk_bits[k]));
   L.partial = take(part);
@@ -337,28 +384,28 @@ int32_t enqueue(const tb_gccb_config &c, char *ws, hipStream_t s) {
   int *count = (int *)(ws + L.count);
   int *outs = (int *)(ws + L.outs);
   tb::cd *Vt = (tb::cd *)(ws + L.vt);
-  tb::cd *minors = (tb::cd *)(ws + L.minors);
   tb::cd *partial = (tb::cd *)(ws + L.partial);
   const int uin_items = c.per_sample ? S : 1;
   hipLaunchKernelGGL(tb::gccb_uin_kernel,
                      dim3((unsigned)(((int64_t)uin_items * m * n + 255) / 256)), dim3(256), 0, s,
                      prm, m, n, uin_items, UinT);
-  hipLaunchKernelGGL(tb::gccb_init_kernel, dim3((S + 255) / 256), dim3(256), 0, s, prm, m, n, S,
-                     order, steps);
+  hipLaunchKernelGGL(tb::gccb_init_kernel, dim3((S + tb::kInitBlock - 1) / tb::kInitBlock),
+                     dim3(tb::kInitBlock), 0, s, prm, m, n, S, order, steps);
   hipLaunchKernelGGL(tb::gccb_order_kernel, dim3(1), dim3(1024), 0, s, steps, S, n, slot_map,
                      count);
   int32_t rc = tbh::check_hip(hipGetLastError(), "gccb_init");
   if (rc) return rc;
   const size_t step_lds =
       (size_t)n * sizeof(tb::cd) + (size_t)m * sizeof(double) + (size_t)n * sizeof(int);
+  int wpi = 0;
   for (int k = 1; k <= n; ++k) {
     if (k >= 2) {
-      rc = tbs::run_sweep(k, Vt, n, (int64_t)n * n, slot_map, count + k, S, c.chunk_bits[k],
-                          partial, minors, s);
+      rc = tbs::run_sweep_partial(k, Vt, n, (int64_t)n * n, slot_map, count + k, S,
+                                  c.chunk_bits[k], partial, &wpi, s);
       if (rc) return rc;
     }
     hipLaunchKernelGGL(tb::gccb_step_kernel, dim3(S), dim3(64), step_lds, s, prm, k, m, n, S, count,
-                       slot_map, minors, UinT, order, outs, Vt);
+                       slot_map, partial, wpi, UinT, order, outs, Vt);
     rc = tbh::check_hip(hipGetLastError(), "gccb_step_kernel");
     if (rc) return rc;
   }

@@ -454,6 +454,16 @@ size_t sweep_partial_bytes(int k, int items, int c_override) {
   return align256((size_t)items * p.wpi * k * sizeof(tb::cd));
 }
 
+// Partial sums only: partial[(item * wpi + w) * k + nu], w < *wpi. A consumer that sums
+// them in w order gets the same minors as run_sweep.
+int32_t run_sweep_partial(int k, const tb::cd *Vt, int ldv, int64_t item_stride,
+                          const int *item_map, const int *item_count, int items, int c_override,
+                          tb::cd *partial, int *wpi, hipStream_t s) {
+  const SweepPlan p = sweep_plan(k, c_override);
+  *wpi = p.wpi;
+  return dispatch_sweep(k, Vt, ldv, item_stride, item_map, item_count, items, p, partial, s);
+}
+
 // Full sweep: partial sums then the fixed-order reduction into minors[items][k].
 int32_t run_sweep(int k, const tb::cd *Vt, int ldv, int64_t item_stride, const int *item_map,
                   const int *item_count, int items, int c_override, tb::cd *partial,
