@@ -14,7 +14,7 @@ import pytest
 
 from theboss_amd import build as B
 
-TUS = ("sweep", "glynn_f64", "grad")
+TUS = ("sweep", "glynn_f64", "grad", "sampler")
 
 
 def _vgprs(tok):

@@ -461,6 +461,33 @@ def test_plain_strategy_equals_batched_with_same_draws(T, ops, oracle):
     assert np.array_equal(out.cpu().numpy(), plain)
 
 
+def test_batched_sampler_minors_equal_separate_sweep(T, ops, oracle):
+    """Steps 1..12 run fused in one launch per sample, later steps as sweep + step
+    launches: the traced minors of every step equal, bit for bit, tb_sweep_minors_batch on
+    the same columns (same chunks, wave sums and summation order)."""
+    m, n, S = 400, 16, 24
+    rng = np.random.default_rng(16)
+    U = oracle.haar_unitary(m, 4016)
+    order, unif = _forced(rng, S, n)
+    out, trace = ops.gccb_sample(dev_c(T, U), dev_i(T, np.arange(n)), S, 1, 1.0, dev_i(T, order),
+                                 T.from_numpy(unif).cuda(), None, trace=True)
+    out, tr = out.cpu().numpy(), trace.cpu().numpy()
+    assert np.array_equal(out, oracle.gccb_sample_batch(U, order, unif))
+    # The picks, from the device's own minors (Laplace expansion and inverse CDF).
+    picks = np.zeros((S, n), dtype=np.int64)
+    for s in range(S):
+        for k in range(1, n + 1):
+            amp = U[:, order[s, :k]] @ tr[s, k - 1, :k]
+            cum = np.cumsum(np.abs(amp) ** 2)
+            picks[s, k - 1] = min(int(np.searchsorted(cum, unif[s, k - 1] * cum[-1], side="right")),
+                                  m - 1)
+    assert np.array_equal(np.stack([np.bincount(p, minlength=m) for p in picks]), out)
+    for k in range(2, n + 1):
+        Vt = np.stack([U[np.ix_(picks[s, :k - 1], order[s, :k])].T for s in range(S)])
+        got = ops.sweep_minors_batch(dev_c(T, Vt), k).cpu().numpy()
+        assert np.array_equal(got, tr[:, k - 1, :k]), k
+
+
 def test_errors_surface_as_exceptions(T, ops):
     from theboss_amd._lib import TbError
 

@@ -27,13 +27,14 @@
 #include <tuple>
 #include <vector>
 
-#include "tb_common.hpp"
+#include "sweep_impl.hpp"
 
 namespace tbs {
 size_t sweep_partial_bytes(int k, int items, int c_override);
 int32_t run_sweep_partial(int k, const tb::cd *Vt, int ldv, int64_t item_stride,
                           const int *item_map, const int *item_count, int items, int c_override,
                           tb::cd *partial, int *wpi, hipStream_t s);
+void sweep_plan_of(int k, int c_override, int *c, int *wpi, int *iters);
 }  // namespace tbs
 
 namespace tb {
@@ -228,52 +229,41 @@ __global__ void gccb_uin_kernel(const GccbParams *__restrict__ prm, int m, int n
   UinT[idx] = U[(size_t)mm * prm->ldu + prm->particles[(size_t)s * n + nu]];
 }
 
-// One wave per active slot at step k: the sweep's wpi partial sums of the k minors (added
-// in w order), Laplace over m modes, inverse-CDF draw, output and column bookkeeping for
-// the next step.
-__global__ __launch_bounds__(64) void gccb_step_kernel(
-    const GccbParams *__restrict__ prm, int k, int m, int n, int S, const int *__restrict__ count,
-    const int *__restrict__ slot_map, const cd *__restrict__ partial, int wpi,
-    const cd *__restrict__ UinT, const int *__restrict__ order, int *__restrict__ outs,
-    cd *__restrict__ Vt) {
-  extern __shared__ char smem[];
+// Laplace expansion, draw and output of step k for sample s (one wave): the k minors are
+// in mi (LDS), rows[nu] is particle nu's UinT row. pm is LDS scratch for m probabilities.
+// KC > 0: k == KC at compile time (the particle loop is unrolled, so a lane's loads of one
+// mode are in flight together). Returns the chosen output mode (wave-uniform).
+template <int KC = 0>
+__device__ __forceinline__ int gccb_draw(const GccbParams *__restrict__ prm, int k, int m, int n,
+                                         int s, const cd *mi, const int *rows, double *pm,
+                                         const cd *__restrict__ UinT, int *__restrict__ outs) {
+  if constexpr (KC > 0) k = KC;
+  const int lane = threadIdx.x;
   const double *forced_uniforms = prm->forced_uniforms;
   int *out_samples = prm->out_samples;
   cd *trace = prm->trace;
-  cd *mi = (cd *)smem;                 // k minors
-  double *pm = (double *)(mi + n);     // m probabilities
-  int *rows = (int *)(pm + m);         // UinT row of each of the k particles
-  const int slot = blockIdx.x;
-  const int lane = threadIdx.x;
-  if (slot >= count[k]) return;
-  const int s = slot_map[slot];
-  const int *ord = order + (size_t)s * n;
-  if (prm->per_sample) UinT += (size_t)s * n * m;
-  if (lane < k) {
-    cd v{1.0, 0.0};
-    if (k > 1) {
-      double re = 0.0, im = 0.0;
-      const cd *p = partial + (size_t)slot * wpi * k + lane;
-      for (int w = 0; w < wpi; ++w) {
-        re += p[(size_t)w * k].re;
-        im += p[(size_t)w * k].im;
-      }
-      v = {re, im};
-    }
-    mi[lane] = v;
-    rows[lane] = ord[lane];
-  }
-  __syncthreads();
   if (trace && lane < k) trace[((size_t)s * n + (k - 1)) * n + lane] = mi[lane];
   // pmf: amp_m = sum_nu U[m][in_nu] minor_nu over the current k particles. Lanes take
   // modes lane, lane + 64, ... so each UinT row is read with coalesced loads.
   for (int mm = lane; mm < m; mm += 64) {
     double re = 0.0, im = 0.0;
-    for (int nu = 0; nu < k; ++nu) {
-      const cd u = UinT[(size_t)rows[nu] * m + mm];
-      const cd v = mi[nu];
-      re += u.re * v.re - u.im * v.im;
-      im += u.re * v.im + u.im * v.re;
+    if constexpr (KC > 0) {
+      cd u[KC];
+#pragma unroll
+      for (int nu = 0; nu < KC; ++nu) u[nu] = UinT[(size_t)rows[nu] * m + mm];
+#pragma unroll
+      for (int nu = 0; nu < KC; ++nu) {
+        const cd v = mi[nu];
+        re += u[nu].re * v.re - u[nu].im * v.im;
+        im += u[nu].re * v.im + u[nu].im * v.re;
+      }
+    } else {
+      for (int nu = 0; nu < k; ++nu) {
+        const cd u = UinT[(size_t)rows[nu] * m + mm];
+        const cd v = mi[nu];
+        re += u.re * v.re - u.im * v.im;
+        im += u.re * v.im + u.im * v.re;
+      }
     }
     pm[mm] = re * re + im * im;
   }
@@ -328,9 +318,152 @@ __global__ __launch_bounds__(64) void gccb_step_kernel(
     outs[(size_t)s * n + (k - 1)] = chosen;
     out_samples[(size_t)s * m + chosen] += 1;
   }
+  return chosen;
+}
+
+// Step-kernel LDS: k minors, m probabilities, n particle rows.
+__host__ __device__ constexpr size_t gccb_step_lds(int m, int n) {
+  return (size_t)n * sizeof(cd) + (size_t)m * sizeof(double) + (size_t)n * sizeof(int);
+}
+
+// One wave per active slot at step k: the sweep's wpi partial sums of the k minors (added
+// in w order), Laplace over m modes, inverse-CDF draw, output and column bookkeeping for
+// the next step.
+__global__ __launch_bounds__(64) void gccb_step_kernel(
+    const GccbParams *__restrict__ prm, int k, int m, int n, int S, const int *__restrict__ count,
+    const int *__restrict__ slot_map, const cd *__restrict__ partial, int wpi,
+    const cd *__restrict__ UinT, const int *__restrict__ order, int *__restrict__ outs,
+    cd *__restrict__ Vt) {
+  extern __shared__ char smem[];
+  cd *mi = (cd *)smem;                 // k minors
+  double *pm = (double *)(mi + n);     // m probabilities
+  int *rows = (int *)(pm + m);         // UinT row of each of the k particles
+  const int slot = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (slot >= count[k]) return;
+  const int s = slot_map[slot];
+  const int *ord = order + (size_t)s * n;
+  if (prm->per_sample) UinT += (size_t)s * n * m;
+  if (lane < k) {
+    cd v{1.0, 0.0};
+    if (k > 1) {
+      double re = 0.0, im = 0.0;
+      const cd *p = partial + (size_t)slot * wpi * k + lane;
+      for (int w = 0; w < wpi; ++w) {
+        re += p[(size_t)w * k].re;
+        im += p[(size_t)w * k].im;
+      }
+      v = {re, im};
+    }
+    mi[lane] = v;
+    rows[lane] = ord[lane];
+  }
+  __syncthreads();
+  const int chosen = gccb_draw(prm, k, m, n, s, mi, rows, pm, UinT, outs);
   // New output row j = k-1 for every input particle: Vt[s][nu][k-1] = U[chosen][in_nu].
   for (int nu = lane; nu < n; nu += 64)
     Vt[((size_t)s * n + nu) * n + (k - 1)] = UinT[(size_t)ord[nu] * m + chosen];
+}
+
+// Steps 1..kmax of a sample in one wave (kmax <= kSmallK): for small k the separate sweep
+// and step launches cost more than their work. Per step the wave sweeps the item's
+// columns (kept in LDS; the same chunks, wave sums and w order as sweep_kernel followed by
+// gccb_step_kernel, so the minors are bit-identical), then draws as gccb_step_kernel does.
+// At the end it writes the sample's first kmax output columns of Vt for the later steps.
+#ifndef TB_GCCB_SMALL_K
+#define TB_GCCB_SMALL_K 12
+#endif
+constexpr int kSmallK = TB_GCCB_SMALL_K;
+struct SmallPlan {
+  int c[kSmallK + 1], wpi[kSmallK + 1], iters[kSmallK + 1];
+};
+
+// LDS of the small kernel: Vl [kSmallK][kSmallK] (Vl[nu * kSmallK + j] = V[nu][j]), Vs
+// [kSmallK * (kSmallK - 1)], Ed [ed_slots * 64], then the step's minors, probabilities and
+// rows (gccb_step_lds), then the chosen mode of each step.
+__host__ __device__ constexpr size_t gccb_small_lds(int m, int n, int ed_slots) {
+  return sizeof(cd) * ((size_t)kSmallK * kSmallK + (size_t)kSmallK * (kSmallK - 1) +
+                       (size_t)ed_slots * 64) +
+         gccb_step_lds(m, n) + sizeof(int) * kSmallK;
+}
+
+template <int K>
+__device__ __forceinline__ void gccb_small_sweep(const cd *Vl, cd *Vs, cd *Ed, cd *mi,
+                                                 const SmallPlan &pl) {
+  constexpr int R = K - 1;
+  constexpr int R0 = (K <= TB_SWEEP_BCAST_MAX_K) ? 3 : 2;
+  constexpr int U = (K - 1 < kSweepU) ? K - 1 : kSweepU;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < K * R; e += 64) {
+    const int nu = e / R, j = e - nu * R;
+    Vs[e] = Vl[nu * kSmallK + j];
+  }
+  __syncthreads();
+  double re = 0.0, im = 0.0;
+  for (int w = 0; w < pl.wpi[K]; ++w) {
+    const cd acc = sweep_wave<K, U, R0>(Vs, Ed, lane, pl.c[K], w, pl.wpi[K], pl.iters[K]);
+    re += acc.re;
+    im += acc.im;
+  }
+  if (lane < K) mi[lane] = {re, im};
+  __syncthreads();
+}
+
+template <int K>
+__device__ __forceinline__ int gccb_small_steps(const GccbParams *__restrict__ prm, int kmax,
+                                                int m, int n, int s, int slot,
+                                                const int *__restrict__ count, cd *Vl, cd *Vs,
+                                                cd *Ed, cd *mi, double *pm, const int *rows,
+                                                int *ch, const cd *__restrict__ UinT,
+                                                int *__restrict__ outs, const SmallPlan &pl) {
+  if constexpr (K > kSmallK) {
+    return K - 1;
+  } else {
+    if (K > kmax || slot >= count[K]) return K - 1;
+    const int lane = threadIdx.x;
+    if constexpr (K == 1) {
+      if (lane == 0) mi[0] = {1.0, 0.0};
+      __syncthreads();
+    } else {
+      gccb_small_sweep<K>(Vl, Vs, Ed, mi, pl);
+    }
+    const int chosen = gccb_draw<K>(prm, K, m, n, s, mi, rows, pm, UinT, outs);
+    if (lane == 0) ch[K - 1] = chosen;
+    for (int nu = lane; nu < kmax; nu += 64)
+      Vl[nu * kSmallK + (K - 1)] = UinT[(size_t)rows[nu] * m + chosen];
+    __syncthreads();
+    return gccb_small_steps<K + 1>(prm, kmax, m, n, s, slot, count, Vl, Vs, Ed, mi, pm, rows, ch,
+                                   UinT, outs, pl);
+  }
+}
+
+__global__ __launch_bounds__(64) void gccb_small_kernel(
+    const GccbParams *__restrict__ prm, int kmax, int m, int n, int ed_slots,
+    const int *__restrict__ count, const int *__restrict__ slot_map, const cd *__restrict__ UinT,
+    const int *__restrict__ order, int *__restrict__ outs, cd *__restrict__ Vt, SmallPlan pl) {
+  extern __shared__ char smem[];
+  cd *Vl = (cd *)smem;
+  cd *Vs = Vl + kSmallK * kSmallK;
+  cd *Ed = Vs + kSmallK * (kSmallK - 1);
+  cd *mi = Ed + (size_t)ed_slots * 64;
+  double *pm = (double *)(mi + n);
+  int *rows = (int *)(pm + m);
+  int *ch = rows + n;
+  const int slot = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (slot >= count[1]) return;
+  const int s = slot_map[slot];
+  const int *ord = order + (size_t)s * n;
+  if (prm->per_sample) UinT += (size_t)s * n * m;
+  for (int nu = lane; nu < n; nu += 64) rows[nu] = ord[nu];
+  __syncthreads();
+  const int done = gccb_small_steps<1>(prm, kmax, m, n, s, slot, count, Vl, Vs, Ed, mi, pm, rows,
+                                       ch, UinT, outs, pl);
+  // Vt[s][nu][j] = U[out_j][in_nu] for the done steps, all n particles.
+  for (int e = lane; e < n * done; e += 64) {
+    const int nu = e / done, j = e - nu * done;
+    Vt[((size_t)s * n + nu) * n + j] = UinT[(size_t)rows[nu] * m + ch[j]];
+  }
 }
 
 // Vt rows follow the sample's particle order: Vt[s][nu][j] uses particle ord[nu]. The
@@ -395,10 +528,29 @@ int32_t enqueue(const tb_gccb_config &c, char *ws, hipStream_t s) {
                      count);
   int32_t rc = tbh::check_hip(hipGetLastError(), "gccb_init");
   if (rc) return rc;
-  const size_t step_lds =
-      (size_t)n * sizeof(tb::cd) + (size_t)m * sizeof(double) + (size_t)n * sizeof(int);
+  const size_t step_lds = tb::gccb_step_lds(m, n);
+  // Steps 1..kmax in one launch where its LDS fits; the separate sweep and step launches
+  // take the rest.
+  int k_first = 1;
+  {
+    const int kmax = std::min(n, tb::kSmallK);
+    tb::SmallPlan pl{};
+    int ed_slots = 0;
+    for (int k = 2; k <= kmax; ++k) {
+      tbs::sweep_plan_of(k, c.chunk_bits[k], &pl.c[k], &pl.wpi[k], &pl.iters[k]);
+      ed_slots = std::max(ed_slots, pl.c[k] - std::min(tb::kSweepU, k - 1));
+    }
+    const size_t small_lds = tb::gccb_small_lds(m, n, ed_slots);
+    if (small_lds <= 64 * 1024) {
+      hipLaunchKernelGGL(tb::gccb_small_kernel, dim3(S), dim3(64), small_lds, s, prm, kmax, m, n,
+                         ed_slots, count, slot_map, UinT, order, outs, Vt, pl);
+      rc = tbh::check_hip(hipGetLastError(), "gccb_small_kernel");
+      if (rc) return rc;
+      k_first = kmax + 1;
+    }
+  }
   int wpi = 0;
-  for (int k = 1; k <= n; ++k) {
+  for (int k = k_first; k <= n; ++k) {
     if (k >= 2) {
       rc = tbs::run_sweep_partial(k, Vt, n, (int64_t)n * n, slot_map, count + k, S,
                                   c.chunk_bits[k], partial, &wpi, s);
@@ -478,8 +630,7 @@ extern "C" int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128
   if (!workspace || workspace_bytes < need)
     return tbh::fail(TB_EWORKSPACE, "gccb: workspace %zu bytes < required %zu", workspace_bytes,
                      need);
-  const size_t step_lds =
-      (size_t)c.n * sizeof(tb::cd) + (size_t)c.m * sizeof(double) + (size_t)c.n * sizeof(int);
+  const size_t step_lds = tb::gccb_step_lds(c.m, c.n);
   if (step_lds > 64 * 1024) return tbh::fail(TB_ERANGE, "gccb: m=%d too large for LDS pmf", c.m);
   char *ws = (char *)workspace;
   const Layout L = layout(c);
