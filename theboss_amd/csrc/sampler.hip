@@ -248,14 +248,21 @@ __device__ __forceinline__ int gccb_draw(const GccbParams *__restrict__ prm, int
   for (int mm = lane; mm < m; mm += 64) {
     double re = 0.0, im = 0.0;
     if constexpr (KC > 0) {
-      cd u[KC];
+      constexpr int G = KC < 16 ? KC : 16;  // loads in flight per group
 #pragma unroll
-      for (int nu = 0; nu < KC; ++nu) u[nu] = UinT[(size_t)rows[nu] * m + mm];
+      for (int nu0 = 0; nu0 < KC; nu0 += G) {
+        cd u[G];
 #pragma unroll
-      for (int nu = 0; nu < KC; ++nu) {
-        const cd v = mi[nu];
-        re += u[nu].re * v.re - u[nu].im * v.im;
-        im += u[nu].re * v.im + u[nu].im * v.re;
+        for (int g = 0; g < G; ++g)
+          if (nu0 + g < KC) u[g] = UinT[(size_t)rows[nu0 + g] * m + mm];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+          if (nu0 + g < KC) {
+            const cd v = mi[nu0 + g];
+            re += u[g].re * v.re - u[g].im * v.im;
+            im += u[g].re * v.im + u[g].im * v.re;
+          }
+        }
       }
     } else {
       for (int nu = 0; nu < k; ++nu) {
@@ -326,14 +333,16 @@ __host__ __device__ constexpr size_t gccb_step_lds(int m, int n) {
   return (size_t)n * sizeof(cd) + (size_t)m * sizeof(double) + (size_t)n * sizeof(int);
 }
 
-// One wave per active slot at step k: the sweep's wpi partial sums of the k minors (added
-// in w order), Laplace over m modes, inverse-CDF draw, output and column bookkeeping for
-// the next step.
+// One wave per active slot at step k = K: the sweep's wpi partial sums of the k minors
+// (added in w order), Laplace over m modes, inverse-CDF draw, output and column
+// bookkeeping for the next step.
+template <int K>
 __global__ __launch_bounds__(64) void gccb_step_kernel(
-    const GccbParams *__restrict__ prm, int k, int m, int n, int S, const int *__restrict__ count,
+    const GccbParams *__restrict__ prm, int m, int n, int S, const int *__restrict__ count,
     const int *__restrict__ slot_map, const cd *__restrict__ partial, int wpi,
     const cd *__restrict__ UinT, const int *__restrict__ order, int *__restrict__ outs,
     cd *__restrict__ Vt) {
+  constexpr int k = K;
   extern __shared__ char smem[];
   cd *mi = (cd *)smem;                 // k minors
   double *pm = (double *)(mi + n);     // m probabilities
@@ -359,7 +368,7 @@ __global__ __launch_bounds__(64) void gccb_step_kernel(
     rows[lane] = ord[lane];
   }
   __syncthreads();
-  const int chosen = gccb_draw(prm, k, m, n, s, mi, rows, pm, UinT, outs);
+  const int chosen = gccb_draw<K>(prm, k, m, n, s, mi, rows, pm, UinT, outs);
   // New output row j = k-1 for every input particle: Vt[s][nu][k-1] = U[chosen][in_nu].
   for (int nu = lane; nu < n; nu += 64)
     Vt[((size_t)s * n + nu) * n + (k - 1)] = UinT[(size_t)ord[nu] * m + chosen];
@@ -506,6 +515,22 @@ Layout layout(const tb_gccb_config &c) {
   return L;
 }
 
+template <int K = 1>
+int32_t launch_step(int k, int S, size_t lds, hipStream_t s, const tb::GccbParams *prm, int m,
+                    int n, const int *count, const int *slot_map, const tb::cd *partial, int wpi,
+                    const tb::cd *UinT, const int *order, int *outs, tb::cd *Vt) {
+  if constexpr (K > TB_SWEEP_MAX_K) {
+    return tbh::fail(TB_ERANGE, "gccb: step %d exceeds TB_SWEEP_MAX_K=%d", k, TB_SWEEP_MAX_K);
+  } else {
+    if (k != K)
+      return launch_step<K + 1>(k, S, lds, s, prm, m, n, count, slot_map, partial, wpi, UinT,
+                                order, outs, Vt);
+    hipLaunchKernelGGL(tb::gccb_step_kernel<K>, dim3(S), dim3(64), lds, s, prm, m, n, S, count,
+                       slot_map, partial, wpi, UinT, order, outs, Vt);
+    return tbh::check_hip(hipGetLastError(), "gccb_step_kernel");
+  }
+}
+
 int32_t enqueue(const tb_gccb_config &c, char *ws, hipStream_t s) {
   const Layout L = layout(c);
   const int S = c.samples, n = c.n, m = c.m;
@@ -556,9 +581,8 @@ int32_t enqueue(const tb_gccb_config &c, char *ws, hipStream_t s) {
                                   c.chunk_bits[k], partial, &wpi, s);
       if (rc) return rc;
     }
-    hipLaunchKernelGGL(tb::gccb_step_kernel, dim3(S), dim3(64), step_lds, s, prm, k, m, n, S, count,
-                       slot_map, partial, wpi, UinT, order, outs, Vt);
-    rc = tbh::check_hip(hipGetLastError(), "gccb_step_kernel");
+    rc = launch_step(k, S, step_lds, s, prm, m, n, count, slot_map, partial, wpi, UinT, order,
+                     outs, Vt);
     if (rc) return rc;
   }
   return TB_OK;
