@@ -191,8 +191,8 @@ def main() -> int:
     # permanents (16 x n=32) as % of the FP64 vector peak (full records under "workloads")
     from theboss_amd import workloads
 
-    wl_grad = workloads.grad(orc)
-    wl_glynn = workloads.glynn(orc, ns=(32,), precisions=("fp64",))
+    wl_grad = workloads.grad(orc, cpu=not args.no_cpu_baseline)
+    wl_glynn = workloads.glynn(orc, ns=(32,), precisions=("fp64",), cpu=not args.no_cpu_baseline)
 
     line = {
         "metric": METRIC,

@@ -20,8 +20,8 @@ How each routine is pinned (tests/test_oracle.py):
                           J_n gives n!, derangements); perm(cA)=c^n perm(A); block
                           diagonal factorises; transpose and row/column permutation
                           invariance.
-* ryser_py, cc_submatrices_py (timing restatements in pure Python)
-                       -- equal the C ryser / cc submatrices on random patterns.
+* glynn_py, ryser_py, cc_submatrices_py (timing restatements in pure Python)
+                       -- equal the C glynn / ryser / cc submatrices on random patterns.
 * ryser / chin-huh     -- equal classic on the nine occupation patterns of the
                           reference's tests/test_bs_permanent_calculators.py:88-122
                           and on random multiplicity patterns.
@@ -45,8 +45,10 @@ How each routine is pinned (tests/test_oracle.py):
 * guan sequence        -- visits every r <= s exactly once, one digit +-1 per step.
 """
 import ctypes
+import functools
 import itertools
 import math
+import operator
 import os
 from typing import List, Sequence, Tuple
 
@@ -166,6 +168,33 @@ def perm_glynn_batch(A, mode: int = 1, threads: int = 0) -> np.ndarray:
 def perm_ryser(U, s, t) -> complex:
     U, s, t = _cm(U), _ci(s), _ci(t)
     return _c(lib().orc_perm_guan(_p(U), U.shape[0], _p(s), _p(t), 0))
+
+
+def perm_glynn_py(A) -> complex:
+    """Pure-Python restatement of the reference's Glynn loop, kept for timing the
+    reference's speed class (bench's Glynn CPU line): delta = 1, column sums
+    sums_j = sum_i M[i][j], acc = prod(sums); per Gray flip i: mult = -mult,
+    delta_i = -delta_i, sums_j += 2 delta_i M[i][j], acc += mult * reduce(mul, sums); then
+    acc / 2^(n-1) (glynn_gray_permanent_calculator.py:41-84). Flip order: reflected Gray
+    code, i = ctz(step). Python complex numbers throughout (the reference's numpy scalars
+    are slower)."""
+    M = np.asarray(A, dtype=np.complex128).tolist()
+    n = len(M)
+    if n == 0:
+        return 1 + 0j
+    delta = [1] * n
+    mult = 1
+    sums = [sum(delta[i] * M[i][j] for i in range(n)) for j in range(n)]
+    acc = mult * functools.reduce(operator.mul, sums, 1)
+    for step in range(1, 1 << (n - 1)):
+        i = (step & -step).bit_length() - 1
+        mult = -mult
+        delta[i] = -delta[i]
+        row = M[i]
+        for j in range(n):
+            sums[j] += 2 * delta[i] * row[j]
+        acc += mult * functools.reduce(operator.mul, sums, 1)
+    return complex(acc / 2 ** (n - 1))
 
 
 def perm_ryser_py(U, s, t) -> complex:

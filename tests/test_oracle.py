@@ -145,6 +145,21 @@ def test_python_cc_walk_equals_c_walk(oracle):
         assert np.max(np.abs(got - oracle.cc_submatrices(U, s, t))) < 1e-12, (s, t)
 
 
+def test_python_glynn_walk_equals_c_walk_and_closed_forms(oracle):
+    """The pure-Python Glynn walk (timing baseline) equals the C walk in the reference's
+    arithmetic on seeded Haar blocks, and the closed forms perm(J_n) = n! and
+    perm(J_n - I_n) = !n (derangements)."""
+    for n in range(1, 11):
+        A = oracle.haar_unitary(2 * n, 500 + n)[:n, :n]
+        want = oracle.perm_glynn_matrix(A, 0)
+        assert abs(oracle.perm_glynn_py(A) - want) <= 1e-12 * max(1.0, abs(want)), n
+    for n in (4, 7, 9):
+        J = np.ones((n, n))
+        assert abs(oracle.perm_glynn_py(J) - math.factorial(n)) < 1e-9 * math.factorial(n)
+        assert abs(oracle.perm_glynn_py(J - np.eye(n)) - oracle.derangements(n)) < 1e-9 * math.factorial(n)
+    assert oracle.perm_glynn_py(np.zeros((0, 0))) == 1
+
+
 def test_python_ryser_walk_on_gcca_shapes(oracle):
     """The pure-Python Ryser walk (timing baseline) on GCC-A shapes: a 0/1 substate of the
     input against a bunched output, over all m digits, equals the C walk."""

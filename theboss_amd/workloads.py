@@ -1,8 +1,10 @@
 """Measured runs of the secondary workloads (single large permanents, differentiation,
 GCC-A, uniform-loss GCC-B, interferometer fitting). Each returns one JSON-able record.
 
-`checker` is the CPU oracle module passed in by the caller (bench.py); it is used only
-to produce seeded inputs and reference values, never for the measured computation.
+`checker` is the CPU oracle module passed in by the caller (bench.py); it produces
+seeded inputs and reference values, and its restatements of the reference's loops are
+timed as the CPU baselines reported beside the GPU figures. It never computes a GPU
+figure.
 """
 import collections
 import json
@@ -39,23 +41,65 @@ def _time(fn, reps=1):
     return e0.elapsed_time(e1) / 1e3 / reps
 
 
-def glynn(checker, ns=(24, 28, 32), precisions=("fp64", "fp32_colsum")):
+def _glynn_blocks(checker, n, B=16):
+    mats = []
+    for b in range(B):
+        seed = 1000 * n + b
+        U = checker.haar_unitary(2 * n, seed)
+        r = np.sort(np.random.default_rng(seed).choice(2 * n, n, replace=False))
+        mats.append(U[r, :n])
+    return np.stack(mats)
+
+
+def _cpu_cores():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def glynn_cpu_rates(checker, n0=24, npy=13):
+    """Per (Gray step x column) costs of the reference's Glynn loop on the host: the
+    oracle's C walk in the reference's arithmetic (mode 0), OpenMP over 16 blocks of
+    n0 x n0 (one block per thread), and the pure-Python walk on one n_py x n_py block.
+    Returns (C thread-seconds, Python seconds) per step-column and the sample notes."""
+    cores = _cpu_cores()
+    A = _glynn_blocks(checker, n0)
+    t0 = time.perf_counter()
+    checker.perm_glynn_batch(A, mode=0, threads=cores)
+    dt = time.perf_counter() - t0
+    c_rate = dt * min(cores, len(A)) / (len(A) * 2 ** (n0 - 1) * n0)
+    Ap = _glynn_blocks(checker, npy, 1)[0]
+    t0 = time.perf_counter()
+    checker.perm_glynn_py(Ap)
+    py_rate = (time.perf_counter() - t0) / (2 ** (npy - 1) * npy)
+    note_c = (f"oracle C Glynn walk in the reference's arithmetic on 16 blocks n={n0} "
+              f"({dt:.2f} s on {min(cores, len(A))} threads), per Gray step and column")
+    note_py = (f"pure-Python restatement of the reference's Glynn loop on one n={npy} block "
+               f"({py_rate * 1e9:.0f} ns per Gray step and column, 1 core)")
+    return c_rate, py_rate, cores, note_c, note_py
+
+
+def glynn(checker, ns=(24, 28, 32), precisions=("fp64", "fp32_colsum"), cpu=True):
     """16 collision-free n x n blocks of Haar(2n) per n in {24, 28, 32}; FP64 against the
-    golden oracle values, FP32 column sums against FP64."""
+    golden oracle values, FP32 column sums against FP64. With cpu, the reference's loop
+    timed on the host (C restatement and pure Python) is scaled to each n: the walk costs
+    a fixed time per Gray step and column."""
     with open(os.path.join(ROOT, "tests", "golden", "glynn_large.json")) as f:
         gold = json.load(f)["cases"]
+    rates = glynn_cpu_rates(checker) if cpu else None
     rows = []
     for n in ns:
         B = 16
-        mats = []
-        for b in range(B):
-            seed = 1000 * n + b
-            U = checker.haar_unitary(2 * n, seed)
-            r = np.sort(np.random.default_rng(seed).choice(2 * n, n, replace=False))
-            mats.append(U[r, :n])
-        A = D.to_device_c128(np.stack(mats))
+        A = D.to_device_c128(_glynn_blocks(checker, n, B))
         fl = B * 8 * n * 2 ** (n - 1)
         rec = {"n": n, "batch": B, "flops": fl}
+        if rates:
+            c_rate, py_rate, cores, note_c, note_py = rates
+            work = B * 2 ** (n - 1) * n  # Gray steps x columns of the batch
+            wall_c = work * c_rate / min(cores, B)
+            wall_py = work * py_rate
+            rec["cpu_baseline"] = {"value": B / wall_c, "unit": "permanents/s", "cores": min(cores, B),
+                                   "kind": "port", "est_seconds": wall_c, "sample": note_c}
+            rec["cpu_python_estimate"] = {"value": B / wall_py, "unit": "permanents/s", "cores": 1,
+                                          "kind": "port", "est_seconds": wall_py, "sample": note_py}
         for prec, name in ((L.TB_PREC_FP64, "fp64"), (L.TB_PREC_FP32_COLSUM, "fp32_colsum")):
             if name not in precisions:
                 continue
@@ -70,17 +114,52 @@ def glynn(checker, ns=(24, 28, 32), precisions=("fp64", "fp32_colsum")):
             errs = [abs(vals[b] - ref[b]) / abs(ref[b]) for b in ref]
             rec[name]["max_rel_err_vs_oracle"] = max(errs)
             rec[name]["checked_against"] = f"{len(errs)} golden oracle values"
+            if rates:
+                rec[name]["gpu_vs_cpu_baseline"] = rec["cpu_baseline"]["est_seconds"] / t
         rows.append(rec)
     return {"workload": "single large permanents (Glynn, 16 per n)", "results": rows}
 
 
-def grad(checker):
-    """64 n=20 blocks of Haar(40): forward permanents + all n^2 minors (backward)."""
+def grad_cpu(checker, mats):
+    """The reference's route to a permanent and all its n^2 minors, timed on the host over
+    one block per thread: the forward by the oracle's C Glynn walk in the reference's
+    arithmetic, the minors by n C&C Ryser submatrices walks (the reference's calculator,
+    one per deleted row: perm(A without row i, column j) for all j)."""
+    import concurrent.futures as cf
+
+    cores = _cpu_cores()
+    sample = mats[:cores]
+    n = sample[0].shape[0]
+
+    def one(A):
+        checker.perm_glynn_matrix(A, 0)
+        ones = np.ones(n, dtype=np.int64)
+        for i in range(n):
+            t = ones.copy()
+            t[i] = 0
+            checker.cc_submatrices(A, ones, t)
+
+    checker.lib()
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+        list(ex.map(one, sample))
+    dt = time.perf_counter() - t0
+    return {"value": len(sample) / dt, "unit": "matrices/s (permanent and all n^2 minors)",
+            "cores": cores, "kind": "port",
+            "sample": f"{len(sample)} of the blocks, one per thread ({dt:.2f} s): the oracle's C "
+                      f"Glynn walk for the permanent and {n} C&C Ryser submatrices walks for the "
+                      f"minors, as the reference computes them"}
+
+
+def grad(checker, cpu=True):
+    """64 n=20 blocks of Haar(40): forward permanents + all n^2 minors (backward). With
+    cpu, the reference's route to the same numbers timed on the host beside it."""
     n, B = 20, 64
     mats = []
     for b in range(B):
         r = np.sort(np.random.default_rng(300 + b).choice(40, n, replace=False))
         mats.append(checker.haar_unitary(40, 300 + b)[r, :n])
+    cpu_rec = grad_cpu(checker, mats) if cpu else None
     from .autograd import permanent
 
     T = D.torch()
@@ -104,6 +183,8 @@ def grad(checker):
     f_bwd_exec = B * ((n + 3) // 4) * 2 ** (n - 1) * (8 * n + 42)
     return {"workload": "permanent + gradient, 64 x n=20",
             "autograd_fwd_bwd_ms": t_pass * 1e3,
+            "cpu_baseline": cpu_rec,
+            "gpu_vs_cpu_baseline": (B / t_pass) / cpu_rec["value"] if cpu_rec else None,
             "frac_fp64_peak": (f_fwd + f_grad) / t_pass / FP64_PEAK,
             "frac_fp64_peak_executed": f_bwd_exec / t_pass / FP64_PEAK,
             "perm_only_ms": t_fwd * 1e3, "perm_and_grad_kernel_ms": t_bwd * 1e3,
