@@ -331,8 +331,8 @@ def test_grad_vs_oracle_minors(T, ops, oracle, n):
 @pytest.mark.parametrize("n,B", [(14, 3), (15, 1), (17, 100), (19, 400), (20, 3), (21, 1), (22, 200)])
 def test_grad_single_sweep_vs_oracle_minors(T, ops, oracle, n, B):
     """The one-sweep-per-column-group gradient (n = 14..26): every minor against the
-    oracle's double-double Glynn minors; column groups of 4 that overlap (n % 4 != 0),
-    one-item and many-item plans (chunk sizes from 2^3 to 2^11)."""
+    oracle's double-double Glynn minors; column groups of 8 that overlap (n % 8 != 0),
+    one-item and many-item plans (chunk sizes from 2^2 to 2^11)."""
     As = np.stack([oracle.haar_unitary(2 * n, 500 + 7 * n + b)[:n, :n] for b in range(B)])
     perm, G = ops.perm_grad_batch(dev_c(T, As))
     perm, G = perm.cpu().numpy(), G.cpu().numpy()

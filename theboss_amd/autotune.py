@@ -81,14 +81,23 @@ def glynn_builtin(n: int, batch: int) -> int:
     return max(2, min(c, 12, nb - 8))
 
 
+def grad_columns(n: int) -> int:
+    """Columns per sweep of the gradient kernel at n (8 for 14 <= n <= 26)."""
+    return 8 if 14 <= n <= 26 else 4
+
+
+def grad_static_bits(n: int) -> int:
+    return 2 if grad_columns(n) == 8 else (4 if n <= 19 else 3)
+
+
 def grad_builtin(n: int, batch: int) -> int:
     nb = n - 1
-    groups = (n + 3) // 4
+    g = grad_columns(n)
+    groups = (n + g - 1) // g
     want = 0
     while (batch * groups << want) < 8192 and want < 30:
         want += 1
-    u = 4 if n <= 19 else 3
-    return min(max(min(nb - 6 - want, 11), u), nb - 7)
+    return min(max(min(nb - 6 - want, 11), grad_static_bits(n)), nb - 7)
 
 
 def _candidates(centre: int, lo: int, hi: int, spread: int) -> List[int]:
@@ -180,7 +189,7 @@ def tune_grad(ns: Iterable[int], batch: int, spread: int = 2, reps: int = 3,
             continue
         A = D.to_device_c128(np.stack([_haar_block(2 * n, 2 * n, seed + 37 * n + b)[:n, :n]
                                        for b in range(batch)]))
-        u = 4 if n <= 19 else 3
+        u = grad_static_bits(n)
         c0 = grad_builtin(n, batch)
         cands = _candidates(c0, u, min(11, n - 8), spread)
         out[n] = _pick("grad", n, cands, c0, lambda: ops.perm_grad_batch(A), reps)

@@ -245,7 +245,7 @@ extern "C" int32_t tb_perm_grad_batch(const tb_complex128 *A, int32_t n, int32_t
   if (!workspace || workspace_bytes < need)
     return tbh::fail(TB_EWORKSPACE, "grad: workspace %zu bytes < required %zu", workspace_bytes,
                      need);
-  if (tbgr::grad_supported(n)) {  // one sweep per group of 4 columns
+  if (tbgr::grad_supported(n)) {  // one sweep per group of 4 or 8 columns
     int32_t rc = tbgr::run_grad(n, (const tb::cd *)A, matrix_stride, batch, c,
                                 (tb::cd *)workspace, (tb::cd *)grad_out, s);
     if (rc) return rc;

@@ -10,13 +10,17 @@
 // sums h_j = (1/2) sum_i delta_i A[i][j] gives perm(A) = 2 sum_delta (prod delta) prod_j h_j.
 // perm is linear in A[i][j], and d h_l / d A[i][j] = delta_i [l == j] / 2, so
 //   G[i][j] = sum_delta (prod delta) delta_i prod_{l != j} h_l.
-// One Gray walk over delta serves every row i and a group of 4 columns j. Per step:
+// One Gray walk over delta serves every row i and a group of G = 4 or 8 columns j. Per
+// step:
 //   * one row flip of the column sums (n complex adds);
-//   * the 4 leave-one-out products P_j = prod_{l != j} h_l: a product tree over the
-//     other n-4 columns and 8 multiplies over the group (n + 3 complex multiplies);
-//   * the running sums T_j += (prod delta) P_j (4 complex adds);
-//   * for the flipped bit i, its corrector E_ij += 2 delta_i(old) T_j (4 complex FMAs).
+//   * the G leave-one-out products P_j = prod_{l != j} h_l: a product tree over the
+//     other n-G columns, suffix products and a prefix chain over the group
+//     (n + 2G - 5 complex multiplies);
+//   * the running sums T_j += (prod delta) P_j (G complex adds);
+//   * for the flipped bit i, its corrector E_ij += 2 delta_i(old) T_j (G complex FMAs).
 // At the end of a walk, sum_t delta_i(t) (prod delta) P_j(t) = E_ij + delta_i(end) T_j.
+// 8-column groups halve the sweeps per matrix for 1.3x the work per step; they hold
+// twice the correctors, so they unroll 2 static bits where 4-column groups unroll 3-4.
 //
 // Work split. A lane walks an aligned chunk of 2^c Gray ranks. Bits below U flip inside
 // the unrolled static block of 2^U steps and keep per-lane correctors in registers. Bits
@@ -26,13 +30,13 @@
 // and the owner lane of row i adds 2 delta_i(old) sum T_j. Bits above the chunk are
 // constant over it and contribute delta_i T_j at the chunk end.
 //
-// Cross-lane sums use permlane swaps and DPP (no LDS traffic) and reduce 8 values at once
-// (transposed: each row of 16 lanes ends with one column's real and imaginary parts), so
-// the output of a wave lives in lane (r, p) = (row of 16 lanes, position): column
-// kRowCol[r] of output rows p and 16 + p.
+// Cross-lane sums use permlane swaps and DPP (no LDS traffic) and reduce 8 values (one
+// slot of 4 columns) at once (transposed: each row of 16 lanes ends with one column's
+// real and imaginary parts), so the output of a wave lives in lane (r, p) = (row of 16
+// lanes, position): per slot q, column 4q + grad_row_col(r) of output rows p and 16 + p.
 //
 // Columns. The item (matrix, group g) stages its matrix in LDS with the group's columns
-// last, so the product structure is static. The last group starts at n - 4 and may
+// last, so the product structure is static. The last group starts at n - G and may
 // overlap its predecessor; the reduction takes each column from its first group.
 #include <algorithm>
 
@@ -50,14 +54,38 @@
 #ifndef TB_GRAD_ATTR
 #define TB_GRAD_ATTR __attribute__((amdgpu_waves_per_eu(2)))
 #endif
+// Groups of 8 columns for TB_GRAD_G8_MIN_N <= n <= TB_GRAD_G8_MAX_N (every n of the
+// single-sweep kernel by default; builds that narrow the range get 4-column groups, which
+// need fewer registers): half the sweeps of 4-column groups at ~1.3x the work per step,
+// so 1.1-1.4x faster (n = 16..26), with
+// TB_GRAD_G8_U static bits (the 8 columns' correctors take the registers of a third
+// bit) and TB_GRAD_G8_B broadcast rows up to n = 24 (none above: fewer spills).
+#ifndef TB_GRAD_G8_MIN_N
+#define TB_GRAD_G8_MIN_N 14
+#endif
+#ifndef TB_GRAD_G8_MAX_N
+#define TB_GRAD_G8_MAX_N 26
+#endif
+#ifndef TB_GRAD_G8_U
+#define TB_GRAD_G8_U 2
+#endif
+#ifndef TB_GRAD_G8_B
+#define TB_GRAD_G8_B 2
+#endif
 
 namespace tb {
 
-constexpr int kGradG = 4;  // columns per group (one per row of 16 lanes)
+constexpr int kGradG = 4;  // columns per slot (one per row of 16 lanes)
+// Columns per group: 4 (one slot) or 8 (two slots).
+template <int N>
+constexpr int grad_g() {
+  return (N >= TB_GRAD_G8_MIN_N && N <= TB_GRAD_G8_MAX_N) ? 8 : 4;
+}
 template <int N>
 constexpr int grad_u() {
-  return N <= TB_GRAD_U4_MAX_N ? 4 : 3;
+  return grad_g<N>() == 8 ? TB_GRAD_G8_U : (N <= TB_GRAD_U4_MAX_N ? 4 : 3);
 }
+inline int grad_g_of(int n) { return (n >= TB_GRAD_G8_MIN_N && n <= TB_GRAD_G8_MAX_N) ? 8 : 4; }
 constexpr int kGradMinN = 14;
 constexpr int kGradMaxN = 26;
 
@@ -90,8 +118,8 @@ __device__ __forceinline__ int grad_row_col(int r) { return (r == 1) ? 2 : (r ==
 
 // Wave totals of x[0..3] (real parts of the 4 columns) and x[4..7] (imaginary parts):
 // lane row r receives column grad_row_col(r) in (zr, zi).
-__device__ __forceinline__ void wave_sum_cols(const double (&xr)[kGradG], const double (&xi)[kGradG],
-                                              double &zr, double &zi) {
+__device__ __forceinline__ void wave_sum_cols(const double *xr, const double *xi, double &zr,
+                                              double &zi) {
   const double y0 = swap32_pair_sum(xr[0], xr[1]);
   const double y1 = swap32_pair_sum(xr[2], xr[3]);
   const double y2 = swap32_pair_sum(xi[0], xi[1]);
@@ -132,21 +160,21 @@ __device__ __forceinline__ double grad_end_coef(int i, int c, uint64_t gend, uin
 // Leave-one-out products over the group columns N-4..N-1: Q = prod of the other
 // columns, prefix chain Q h_{N-4} ... and suffix products h_{k+1} ... h_{N-1} (8 complex
 // multiplies besides Q; a two-level tree variant measured slower).
-template <int N>
+template <int N, int G>
 __device__ __forceinline__ void grad_loo(const double (&hr)[N], const double (&hi)[N],
-                                         double (&pr)[kGradG], double (&pi)[kGradG]) {
-  constexpr int NO = N - kGradG;
+                                         double (&pr)[G], double (&pi)[G]) {
+  constexpr int NO = N - G;
   cd q = TreeProd<0, NO>::run(hr, hi);
-  cd suf[kGradG];
-  suf[kGradG - 2] = {hr[N - 1], hi[N - 1]};
+  cd suf[G];
+  suf[G - 2] = {hr[N - 1], hi[N - 1]};
 #pragma unroll
-  for (int k = kGradG - 3; k >= 0; --k) suf[k] = cmul({hr[NO + k + 1], hi[NO + k + 1]}, suf[k + 1]);
+  for (int k = G - 3; k >= 0; --k) suf[k] = cmul({hr[NO + k + 1], hi[NO + k + 1]}, suf[k + 1]);
 #pragma unroll
-  for (int k = 0; k < kGradG; ++k) {
-    cd p = (k == kGradG - 1) ? q : cmul(q, suf[k]);
+  for (int k = 0; k < G; ++k) {
+    cd p = (k == G - 1) ? q : cmul(q, suf[k]);
     pr[k] = p.re;
     pi[k] = p.im;
-    if (k + 1 < kGradG) q = cmul(q, {hr[NO + k], hi[NO + k]});
+    if (k + 1 < G) q = cmul(q, {hr[NO + k], hi[NO + k]});
   }
 }
 
@@ -159,24 +187,24 @@ struct GradRows {
 
 // T[k] += SIGN * P_k for the 4 leave-one-out products, each product's last multiply fused
 // into the accumulation (grad_loo followed by the adds, 6 instructions fewer).
-template <int SIGN, int N>
+template <int SIGN, int N, int G>
 __device__ __forceinline__ void grad_loo_acc(const double (&hr)[N], const double (&hi)[N],
-                                             double (&Tr)[kGradG], double (&Ti)[kGradG]) {
-  constexpr int NO = N - kGradG;
+                                             double (&Tr)[G], double (&Ti)[G]) {
+  constexpr int NO = N - G;
   cd q = TreeProd<0, NO>::run(hr, hi);
-  cd suf[kGradG];
-  suf[kGradG - 2] = {hr[N - 1], hi[N - 1]};
+  cd suf[G];
+  suf[G - 2] = {hr[N - 1], hi[N - 1]};
 #pragma unroll
-  for (int k = kGradG - 3; k >= 0; --k) suf[k] = cmul({hr[NO + k + 1], hi[NO + k + 1]}, suf[k + 1]);
+  for (int k = G - 3; k >= 0; --k) suf[k] = cmul({hr[NO + k + 1], hi[NO + k + 1]}, suf[k + 1]);
 #pragma unroll
-  for (int k = 0; k < kGradG; ++k) {
-    if (k == kGradG - 1) {
+  for (int k = 0; k < G; ++k) {
+    if (k == G - 1) {
       Tr[k] += SIGN * q.re;
       Ti[k] += SIGN * q.im;
     } else {
       cmul_acc<SIGN>(Tr[k], Ti[k], q, suf[k]);
     }
-    if (k + 1 < kGradG) q = cmul(q, {hr[NO + k], hi[NO + k]});
+    if (k + 1 < G) q = cmul(q, {hr[NO + k], hi[NO + k]});
   }
 }
 
@@ -197,18 +225,20 @@ __device__ __forceinline__ void grad_flip(double (&hr)[N], double (&hi)[N], cons
 }
 
 // One wave per workgroup; blockIdx = item * wpi + w, item = matrix * ngroups + group.
-// partial[(item * wpi + w) * N * 4 + row * 4 + col].
-template <int N, int U>
+// partial[(item * wpi + w) * N * G + row * G + col]. G = 4 Q columns in Q slots of 4.
+template <int N, int U, int G>
 __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restrict__ A,
                                                                int64_t mstride, int ngroups, int c,
                                                                int wpi, int iters,
                                                                cd *__restrict__ partial) {
-  constexpr int G = kGradG;
+  constexpr int Q = G / kGradG;
   constexpr int NB = N - 1;
   constexpr int NO = N - G;
-  // broadcast rows (register budget): all 3 static rows up to n = 22, 2 up to 24, then 1; the
-  // 4-bit blocks of n <= 19 keep row 0 in SGPRs
-  constexpr int B = (U == 3) ? (N <= 22 ? 3 : N <= 24 ? 2 : 1) : 0;
+  // broadcast rows (register budget): with 4 columns all 3 static rows up to n = 22, 2 up
+  // to 24, then 1 (the 4-bit blocks of n <= 19 keep row 0 in SGPRs); with 8 columns
+  // TB_GRAD_G8_B of the static rows
+  constexpr int B = (G == 8) ? (N > 24 ? 0 : TB_GRAD_G8_B < U ? TB_GRAD_G8_B : U)
+                             : (U == 3) ? (N <= 22 ? 3 : N <= 24 ? 2 : 1) : 0;
   constexpr int S = B > 0 ? 0 : ((TB_GRAD_SROW < N) ? TB_GRAD_SROW : N);
   __shared__ cpx<double> Ms[N * N];
   const int lane = threadIdx.x;
@@ -235,7 +265,9 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
     gb.one = vgpr_one();
     asm volatile("s_nop 4");  // VALU writes of the store before its first DPP read
   }
-  GradAcc acc = {0.0, 0.0, 0.0, 0.0};
+  GradAcc acc[Q];
+#pragma unroll
+  for (int qq = 0; qq < Q; ++qq) acc[qq] = {0.0, 0.0, 0.0, 0.0};
   const int nblocks = 1 << (c - U);
   for (int it = 0; it < iters; ++it) {
     const int wt = w + it * wpi;
@@ -260,7 +292,7 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
       }
     }
     double Tr[G], Ti[G];
-    grad_loo<N>(hr, hi, Tr, Ti);  // prod(delta) = (-1)^t0 = +1 (t0 even)
+    grad_loo<N, G>(hr, hi, Tr, Ti);  // prod(delta) = (-1)^t0 = +1 (t0 even)
     double Er[U][G], Ei[U][G];
 #pragma unroll
     for (int i = 0; i < U; ++i)
@@ -303,9 +335,9 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
           grad_flip<N, S, 0, B>(hr, hi, sr, gb, M, i, sg);
         }
         if (jj & 1)
-          grad_loo_acc<-1, N>(hr, hi, Tr, Ti);
+          grad_loo_acc<-1, N, G>(hr, hi, Tr, Ti);
         else
-          grad_loo_acc<1, N>(hr, hi, Tr, Ti);
+          grad_loo_acc<1, N, G>(hr, hi, Tr, Ti);
         TB_STEP_FENCE();
       }
     };
@@ -314,13 +346,14 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
       const int i = U + __builtin_ctz(blk);
       const bool t_next = (i + 1 < c) ? (((blk >> (i + 1 - U)) & 1) != 0) : q_odd;
       const double sg = t_next ? 1.0 : -1.0;
-      {  // wave-level corrector of bit i
+#pragma unroll
+      for (int qq = 0; qq < Q; ++qq) {  // wave-level corrector of bit i
         double zr, zi;
-        wave_sum_cols(Tr, Ti, zr, zi);
-        grad_acc_row(acc, p, i, -2.0 * sg, zr, zi);
+        wave_sum_cols(Tr + kGradG * qq, Ti + kGradG * qq, zr, zi);
+        grad_acc_row(acc[qq], p, i, -2.0 * sg, zr, zi);
       }
       flip_mem<N, 0, double>(hr, hi, M + (size_t)i * N, sg);
-      grad_loo_acc<1, N>(hr, hi, Tr, Ti);
+      grad_loo_acc<1, N, G>(hr, hi, Tr, Ti);
       TB_STEP_FENCE();
       static_steps(blk);
     }
@@ -336,20 +369,26 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
         xr[k] = fma(d, Tr[k], Er[i][k]);
         xi[k] = fma(d, Ti[k], Ei[i][k]);
       }
-      double zr, zi;
-      wave_sum_cols(xr, xi, zr, zi);
-      grad_acc_row(acc, p, i, 1.0, zr, zi);
+#pragma unroll
+      for (int qq = 0; qq < Q; ++qq) {
+        double zr, zi;
+        wave_sum_cols(xr + kGradG * qq, xi + kGradG * qq, zr, zi);
+        grad_acc_row(acc[qq], p, i, 1.0, zr, zi);
+      }
     }
     {  // rows whose delta is wave-uniform at the end: in-chunk bits >= U, bits above the
        // lane bits, and the fixed row N-1
-      double zr, zi;
-      wave_sum_cols(Tr, Ti, zr, zi);
       const double f0 = grad_end_coef<NB, U>(p, c, gend, g0);
       const double f1 = grad_end_coef<NB, U>(16 + p, c, gend, g0);
-      acc.r0 = fma(f0, zr, acc.r0);
-      acc.i0 = fma(f0, zi, acc.i0);
-      acc.r1 = fma(f1, zr, acc.r1);
-      acc.i1 = fma(f1, zi, acc.i1);
+#pragma unroll
+      for (int qq = 0; qq < Q; ++qq) {
+        double zr, zi;
+        wave_sum_cols(Tr + kGradG * qq, Ti + kGradG * qq, zr, zi);
+        acc[qq].r0 = fma(f0, zr, acc[qq].r0);
+        acc[qq].i0 = fma(f0, zi, acc[qq].i0);
+        acc[qq].r1 = fma(f1, zr, acc[qq].r1);
+        acc[qq].i1 = fma(f1, zi, acc[qq].i1);
+      }
     }
     // Rows c .. c+6: delta_i depends on the lane (chunk index bits 1..6).
     const int iend = min(c + 7, NB);
@@ -362,31 +401,37 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
         xr[k] = d * Tr[k];
         xi[k] = d * Ti[k];
       }
-      double zr, zi;
-      wave_sum_cols(xr, xi, zr, zi);
-      grad_acc_row(acc, p, i, 1.0, zr, zi);
+#pragma unroll
+      for (int qq = 0; qq < Q; ++qq) {
+        double zr, zi;
+        wave_sum_cols(xr + kGradG * qq, xi + kGradG * qq, zr, zi);
+        grad_acc_row(acc[qq], p, i, 1.0, zr, zi);
+      }
     }
   }
-  const int col = grad_row_col(lane >> 4);
   cd *dst = partial + ((size_t)item * wpi + w) * N * G;
-  if (p < N) dst[(size_t)p * G + col] = {acc.r0, acc.i0};
-  if (16 + p < N) dst[(size_t)(16 + p) * G + col] = {acc.r1, acc.i1};
+#pragma unroll
+  for (int qq = 0; qq < Q; ++qq) {
+    const int col = kGradG * qq + grad_row_col(lane >> 4);
+    if (p < N) dst[(size_t)p * G + col] = {acc[qq].r0, acc[qq].i0};
+    if (16 + p < N) dst[(size_t)(16 + p) * G + col] = {acc[qq].r1, acc[qq].i1};
+  }
 }
 
 // grad[b][i][j] = sum_w partial[(b, owner group of j), w][i][j - j0], fixed order.
-__global__ void grad_reduce_kernel(const cd *__restrict__ partial, int n, int ngroups, int wpi,
-                                   int batch, cd *__restrict__ grad) {
+__global__ void grad_reduce_kernel(const cd *__restrict__ partial, int n, int G, int ngroups,
+                                   int wpi, int batch, cd *__restrict__ grad) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)batch * n * n) return;
   const int b = (int)(idx / ((int64_t)n * n));
   const int rem = (int)(idx - (int64_t)b * n * n);
   const int i = rem / n, j = rem - i * n;
-  const int g = min(j / kGradG, ngroups - 1);
-  const int k = j - min(g * kGradG, n - kGradG);
-  const cd *src = partial + ((size_t)(b * ngroups + g) * wpi) * n * kGradG + (size_t)i * kGradG + k;
+  const int g = min(j / G, ngroups - 1);
+  const int k = j - min(g * G, n - G);
+  const cd *src = partial + ((size_t)(b * ngroups + g) * wpi) * n * G + (size_t)i * G + k;
   double re = 0.0, im = 0.0;
   for (int w = 0; w < wpi; ++w) {
-    const cd v = src[(size_t)w * n * kGradG];
+    const cd v = src[(size_t)w * n * G];
     re += v.re;
     im += v.im;
   }
@@ -406,7 +451,8 @@ struct GradPlan {
 // index keeps >= 7 bits (6 lane bits and the wave's bit 0).
 inline GradPlan grad_plan(int n, int batch, int c_pref) {
   GradPlan p{};
-  p.ngroups = (n + tb::kGradG - 1) / tb::kGradG;
+  const int G = tb::grad_g_of(n);
+  p.ngroups = (n + G - 1) / G;
   const int nb = n - 1;
   const int64_t items = (int64_t)batch * p.ngroups;
   int c;
@@ -417,7 +463,7 @@ inline GradPlan grad_plan(int n, int batch, int c_pref) {
     while ((items << want_bits) < 8192 && want_bits < 30) ++want_bits;
     c = std::min(nb - 6 - want_bits, 11);
   }
-  c = std::max(c, n <= TB_GRAD_U4_MAX_N ? 4 : 3);
+  c = std::max(c, G == 8 ? TB_GRAD_G8_U : (n <= TB_GRAD_U4_MAX_N ? 4 : 3));
   c = std::min(c, nb - 7);
   p.c = c;
   const int task_bits = nb - c - 6;  // >= 1
@@ -434,7 +480,7 @@ bool grad_supported(int n) { return n >= tb::kGradMinN && n <= tb::kGradMaxN; }
 size_t grad_partial_bytes(int n, int batch, int c_pref) {
   if (!grad_supported(n) || batch <= 0) return 0;
   const GradPlan p = grad_plan(n, batch, c_pref);
-  return align256((size_t)batch * p.ngroups * p.wpi * n * tb::kGradG * sizeof(tb::cd));
+  return align256((size_t)batch * p.ngroups * p.wpi * n * tb::grad_g_of(n) * sizeof(tb::cd));
 }
 
 template <int N = tb::kGradMinN>
@@ -445,8 +491,8 @@ int32_t dispatch_grad(int n, const tb::cd *A, int64_t mstride, int batch, const 
   } else {
     if (n != N) return dispatch_grad<N + 1>(n, A, mstride, batch, p, partial, s);
     const unsigned grid = (unsigned)((int64_t)batch * p.ngroups * p.wpi);
-    hipLaunchKernelGGL((tb::grad_kernel<N, tb::grad_u<N>()>), dim3(grid), dim3(64), 0, s, A, mstride,
-                       p.ngroups, p.c, p.wpi, p.iters, partial);
+    hipLaunchKernelGGL((tb::grad_kernel<N, tb::grad_u<N>(), tb::grad_g<N>()>), dim3(grid), dim3(64),
+                       0, s, A, mstride, p.ngroups, p.c, p.wpi, p.iters, partial);
     return tbh::check_hip(hipGetLastError(), "grad_kernel");
   }
 }
@@ -459,7 +505,7 @@ int32_t run_grad(int n, const tb::cd *A, int64_t mstride, int batch, int c_pref,
   if (rc) return rc;
   const int64_t total = (int64_t)batch * n * n;
   hipLaunchKernelGGL(tb::grad_reduce_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                     s, partial, n, p.ngroups, p.wpi, batch, grad);
+                     s, partial, n, tb::grad_g_of(n), p.ngroups, p.wpi, batch, grad);
   return tbh::check_hip(hipGetLastError(), "grad_reduce_kernel");
 }
 

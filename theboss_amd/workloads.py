@@ -151,15 +151,22 @@ def grad_cpu(checker, mats):
                       f"minors, as the reference computes them"}
 
 
-def grad(checker, cpu=True):
+def grad(checker, cpu=True, tune=True):
     """64 n=20 blocks of Haar(40): forward permanents + all n^2 minors (backward). With
-    cpu, the reference's route to the same numbers timed on the host beside it."""
+    tune, the gradient kernel's chunk size is first tuned for this shape (as the sampler's
+    are for theirs); with cpu, the reference's route to the same numbers is timed on the
+    host beside it."""
     n, B = 20, 64
     mats = []
     for b in range(B):
         r = np.sort(np.random.default_rng(300 + b).choice(40, n, replace=False))
         mats.append(checker.haar_unitary(40, 300 + b)[r, :n])
     cpu_rec = grad_cpu(checker, mats) if cpu else None
+    tuned = None
+    if tune:
+        from . import autotune
+
+        tuned = autotune.tune_grad([n], B, reps=3)[n]
     from .autograd import permanent
 
     T = D.torch()
@@ -178,12 +185,15 @@ def grad(checker, cpu=True):
     # All n^2 minors by one sweep: 2^(n-1) (2n^2 + 20n) (n column adds, ~3n multiplies for
     # leave-one-out products, n^2 signed accumulations per step).
     f_grad = B * 2 ** (n - 1) * (2 * n * n + 20 * n)
-    # Executed: ceil(n/4) sweeps over groups of 4 columns, each step a row flip (2n), n + 3
-    # complex multiplies (6 each), 4 running sums (2 each) and 4 corrector FMAs (4 each).
-    f_bwd_exec = B * ((n + 3) // 4) * 2 ** (n - 1) * (8 * n + 42)
+    # Executed: ceil(n/G) sweeps over groups of G = 8 columns, each step a row flip (2n),
+    # n + 2G - 5 complex multiplies (6 each), G running sums (2 each) and G corrector FMAs
+    # (4 each): 8n + 18G - 30 flops.
+    G = 8
+    f_bwd_exec = B * ((n + G - 1) // G) * 2 ** (n - 1) * (8 * n + 18 * G - 30)
     return {"workload": "permanent + gradient, 64 x n=20",
             "autograd_fwd_bwd_ms": t_pass * 1e3,
             "cpu_baseline": cpu_rec,
+            "autotune": tuned,
             "gpu_vs_cpu_baseline": (B / t_pass) / cpu_rec["value"] if cpu_rec else None,
             "frac_fp64_peak": (f_fwd + f_grad) / t_pass / FP64_PEAK,
             "frac_fp64_peak_executed": f_bwd_exec / t_pass / FP64_PEAK,
@@ -191,7 +201,7 @@ def grad(checker, cpu=True):
             "flops_counted": f_fwd + f_grad, "flops_executed": f_bwd_exec,
             "note": "autograd pass: the forward of permanent() on a tensor that requires grad "
                     "runs the gradient launch, which returns perm and all n^2 minors (one Gray "
-                    "sweep per group of 4 columns); the backward scales the kept minors. "
+                    "sweep per group of 8 columns); the backward scales the kept minors. "
                     "Counted flops: Glynn forward 8n 2^(n-1) plus the single-sweep all-minors "
                     "count 2^(n-1)(2n^2 + 20n), per matrix"}
 
