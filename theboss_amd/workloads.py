@@ -166,7 +166,7 @@ def grad(checker, cpu=True, tune=True):
     if tune:
         from . import autotune
 
-        tuned = autotune.tune_grad([n], B, reps=3)[n]
+        tuned = autotune.tune_grad([n], B, spread=4, reps=3)[n]
     from .autograd import permanent
 
     T = D.torch()
