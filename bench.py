@@ -50,11 +50,11 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in chunk sizes instead of tuning the sweeps first")
     ap.add_argument("--workload", default="gccb",
-                    choices=["gccb", "glynn", "grad", "gcca", "lossy", "fit", "hist", "bobs"],
+                    choices=["gccb", "glynn", "grad", "gcca", "lossy", "fit", "hist", "bobs", "exact"],
                     help="gccb: the benchmark line (default); the others measure the "
                          "single-large-permanent, differentiation, GCC-A, uniform-loss, "
-                         "interferometer-fit, outcome-histogram and BOBS workloads and print "
-                         "their own JSON line")
+                         "interferometer-fit, outcome-histogram, BOBS and exact-distribution "
+                         "workloads and print their own JSON line")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only run the k=20 sweep twice and a 256 MiB device copy (the "
                          "calibration reference) for rocprofv3 --pmc passes")

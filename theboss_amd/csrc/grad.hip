@@ -430,7 +430,8 @@ __global__ void grad_reduce_kernel(const cd *__restrict__ partial, int n, int G,
   const int k = j - min(g * G, n - G);
   const cd *src = partial + ((size_t)(b * ngroups + g) * wpi) * n * G + (size_t)i * G + k;
   double re = 0.0, im = 0.0;
-  for (int w = 0; w < wpi; ++w) {
+#pragma unroll 8
+  for (int w = 0; w < wpi; ++w) {  // unrolled: the loads are in flight together
     const cd v = src[(size_t)w * n * G];
     re += v.re;
     im += v.im;

@@ -358,7 +358,8 @@ __global__ __launch_bounds__(64) void gccb_step_kernel(
     if (k > 1) {
       double re = 0.0, im = 0.0;
       const cd *p = partial + (size_t)slot * wpi * k + lane;
-      for (int w = 0; w < wpi; ++w) {
+#pragma unroll 8
+      for (int w = 0; w < wpi; ++w) {  // unrolled: the loads are in flight together
         re += p[(size_t)w * k].re;
         im += p[(size_t)w * k].im;
       }

@@ -54,7 +54,8 @@ __global__ void sweep_reduce_kernel(const cd *__restrict__ partial, int K, int w
   const int b = idx / K, nu = idx - b * K;
   if (item_count && b >= *item_count) return;
   double re = 0.0, im = 0.0;
-  for (int w = 0; w < wpi; ++w) {
+#pragma unroll 8
+  for (int w = 0; w < wpi; ++w) {  // unrolled: the loads are in flight together
     const cd v = partial[((size_t)b * wpi + w) * K + nu];
     re += v.re;
     im += v.im;

@@ -48,35 +48,61 @@ class BSDistributionCalculatorWithFixedLosses:
         return self.calculate_probabilities_of_outcomes(self.get_outcomes_in_proper_order())
 
     def calculate_probabilities_of_outcomes(self, outcomes: Iterable[Iterable[int]]) -> List[float]:
-        outcomes = [tuple(int(x) for x in as_occupation(o)) for o in outcomes]
         return list(_fixed_losses_probabilities(self.configuration, self._permanent_calculator,
-                                                outcomes))
+                                                list(outcomes)))
+
+
+def _as_outcome_array(outcomes, m: int) -> np.ndarray:
+    """Outcomes as a B x m int64 array (the per-vector checks of as_occupation, at once)."""
+    a = np.asarray(outcomes, dtype=np.float64).reshape(-1, m) if len(outcomes) else \
+        np.zeros((0, m))
+    r = np.round(a)
+    if not np.all(r == a):
+        raise ValueError("occupation numbers must be integers")
+    if np.any(r < 0):
+        raise ValueError("occupation numbers must be non-negative")
+    return r.astype(np.int64)
 
 
 def _fixed_losses_probabilities(cfg, calc, outcomes) -> np.ndarray:
-    """One batched permanent launch for all (outcome, lossy input) pairs."""
+    """bs_distribution_calculator_with_fixed_losses.py:108-151 for many outcomes: per
+    particle number, every (outcome, lossy input) permanent in one launch (all pairs,
+    gathered on the device, when the calculator offers it), then the reference's weights
+    as array arithmetic."""
     initial = [int(x) for x in as_occupation(cfg.initial_state)]
-    n0 = sum(initial)
+    n0, m = sum(initial), len(initial)
     calc.matrix = cfg.interferometer_matrix
-    groups = {}
-    for idx, t in enumerate(outcomes):
-        groups.setdefault(sum(t), []).append(idx)
-    result = np.zeros(len(outcomes))
-    for left, idxs in groups.items():
-        lossy = generate_lossy_n_particle_input_states(initial, left)
-        mult = np.array([math.prod(math.comb(initial[i], initial[i] - s[i]) for i in range(len(s)))
+    outs = _as_outcome_array(outcomes, m)
+    result = np.zeros(len(outs))
+    if len(outs) == 0:
+        return result
+    fact = np.array([float(math.factorial(i)) for i in range(max(n0, int(outs.max())) + 1)])
+    init = np.asarray(initial, dtype=np.int64)
+    left_of = outs.sum(axis=1)
+    for left in np.unique(left_of):
+        left = int(left)
+        idxs = np.nonzero(left_of == left)[0]
+        lossy = np.asarray(generate_lossy_n_particle_input_states(initial, left),
+                           dtype=np.int64).reshape(-1, m)
+        mult = np.array([math.prod(math.comb(int(a), int(a - b)) for a, b in zip(init, s))
                          for s in lossy], dtype=np.float64)
-        norm = np.array([math.prod(math.factorial(c) for c in s) for s in lossy], dtype=np.float64)
-        ins = [s for _ in idxs for s in lossy]
-        outs = [outcomes[i] for i in idxs for _ in lossy]
-        perms = calc.compute_permanent_batch(ins, outs).reshape(len(idxs), len(lossy))
-        sub = (perms.real ** 2 + perms.imag ** 2) / norm[None, :] * mult[None, :]
-        p = sub.sum(axis=1)
-        p /= math.factorial(sum(lossy[-1]))
+        norm = np.prod(fact[lossy], axis=1)
+        if left == 0:
+            p = np.full(len(idxs), float(np.sum(mult / norm)))
+        else:
+            perms = calc.compute_permanent_outer(lossy, outs[idxs]) \
+                if hasattr(calc, "compute_permanent_outer") else None
+            if perms is not None:  # (outcomes x lossy inputs) device tensor
+                sq = (perms.real ** 2 + perms.imag ** 2).cpu().numpy()
+            else:
+                ins = [tuple(s) for _ in idxs for s in lossy]
+                pairs_out = [tuple(outs[i]) for i in idxs for _ in lossy]
+                pr = calc.compute_permanent_batch(ins, pairs_out).reshape(len(idxs), len(lossy))
+                sq = pr.real ** 2 + pr.imag ** 2
+            p = (sq / norm[None, :] * mult[None, :]).sum(axis=1)
+        p /= math.factorial(int(lossy[-1].sum()))
         p /= math.comb(n0, left)
-        for row, i in enumerate(idxs):
-            t = outcomes[i]
-            result[i] = p[row] * math.factorial(left) / math.prod(math.factorial(c) for c in t)
+        result[idxs] = p * math.factorial(left) / np.prod(fact[outs[idxs]], axis=1)
     return result
 
 
@@ -101,17 +127,18 @@ class BSDistributionCalculatorWithUniformLosses(BSDistributionCalculatorWithFixe
                                         self.configuration.number_of_modes, losses=True)
 
     def calculate_probabilities_of_outcomes(self, outcomes: Iterable[Iterable[int]]) -> List[float]:
-        outcomes = [tuple(int(x) for x in as_occupation(o)) for o in outcomes]
-        probs = np.zeros(len(outcomes))
-        nonempty = [i for i, t in enumerate(outcomes) if sum(t) > 0]
-        for i, t in enumerate(outcomes):
-            if sum(t) == 0:
-                probs[i] = self.weights[0]
-        if nonempty:
+        """bs_exact_distribution_with_uniform_losses.py:62-97: the empty outcome has weight
+        eta^0 C(n, 0) (1 - eta)^n; the others are fixed-loss probabilities weighted by
+        their particle number's binomial weight."""
+        outs = _as_outcome_array(list(outcomes), self.configuration.number_of_modes)
+        probs = np.zeros(len(outs))
+        left = outs.sum(axis=1)
+        probs[left == 0] = self.weights[0]
+        nonempty = np.nonzero(left > 0)[0]
+        if len(nonempty):
             fixed = _fixed_losses_probabilities(self.configuration, self._permanent_calculator,
-                                                [outcomes[i] for i in nonempty])
-            for j, i in enumerate(nonempty):
-                probs[i] = fixed[j] * self.weights[sum(outcomes[i])]
+                                                outs[nonempty])
+            probs[nonempty] = fixed * np.asarray(self.weights, dtype=np.float64)[left[nonempty]]
         return list(probs)
 
 

@@ -206,6 +206,48 @@ def grad(checker, cpu=True, tune=True):
                     "count 2^(n-1)(2n^2 + 20n), per matrix"}
 
 
+def exact(checker, m=12, n=6, eta=0.5):
+    """Exact uniform-loss output distribution, m = 12, n = 6 photons in modes 0..5,
+    eta = 0.5: all C(m + n, n) = 18,564 outcomes with at most n photons, every
+    (outcome, lossy input) permanent by one batched Ryser launch per particle number.
+    CPU line: the oracle's restatement of the reference's per-outcome loop
+    (bs_exact_distribution_with_uniform_losses.py), C permanents, one core."""
+    from .distribution_calculators import (BosonSamplingExperimentConfiguration,
+                                           BSDistributionCalculatorWithUniformLosses)
+    from .permanent_calculators import GPURyserPermanentCalculator
+
+    U = checker.haar_unitary(m, 1200 + m)
+    inp = [1] * n + [0] * (m - n)
+    cfg = BosonSamplingExperimentConfiguration(U, inp, n, m, 0, n, eta)
+
+    def run():
+        calc = BSDistributionCalculatorWithUniformLosses(cfg, GPURyserPermanentCalculator(U))
+        return np.asarray(calc.calculate_distribution())
+
+    T = D.torch()
+    run()  # warm (library, workspaces)
+    T.cuda.synchronize()
+    t0 = time.perf_counter()
+    dist = run()
+    T.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    perms = sum(math.comb(l + m - 1, l) * math.comb(n, l) for l in range(1, n + 1))
+    t0 = time.perf_counter()
+    ref = np.asarray(checker.uniform_losses_distribution(U, inp, eta, checker.perm_ryser))
+    dt_cpu = time.perf_counter() - t0
+    return {"workload": f"exact uniform-loss distribution, m={m}, n={n}, eta={eta}",
+            "outcomes": len(dist), "permanents": perms, "seconds": dt,
+            "permanents_per_s": perms / dt, "sum": float(dist.sum()),
+            "max_abs_diff_vs_oracle": float(np.max(np.abs(dist - ref))),
+            "cpu_baseline": {"value": perms / dt_cpu, "unit": "permanents/s", "cores": 1,
+                             "kind": "port", "seconds": dt_cpu,
+                             "sample": "the whole distribution by the oracle's per-outcome loop "
+                                       "(C Ryser permanents, Python loop), 1 core"},
+            "gpu_vs_cpu_baseline": dt_cpu / dt,
+            "note": "end to end from Python: outcome enumeration, host index lists and the "
+                    "per-particle-number launches"}
+
+
 def gcca(checker):
     """GCC-A, m=20, n=10 photons in modes 0..9, 200 samples, Ryser on the GPU (the
     samples advance in lock-step: one all-pairs launch per particle number); plus one
