@@ -359,12 +359,61 @@ def hist(checker):
                                                        None, stream)), reps=3)
     read = S * m * 4
     algo = read + S * 4  # one 4-byte counter update per sample
+    cpu, cpu_py = _hist_cpu_lines(checker, samples[: 1 << 20].cpu().numpy(), counts, n, m, S)
     return {"workload": "outcome histogram, 2^24 samples, m=20, n=10 (20,030,010 outcomes)",
             "ms": t * 1e3, "samples_per_s": S / t,
             "algorithmic_bytes": algo, "achieved_GBps": algo / t / 1e9,
             "frac_hbm_peak": algo / t / 8e12,
             "counted": int(counts.sum().item()), "outside": outside,
-            "distinct_outcomes_hit": int((counts > 0).sum().item())}
+            "distinct_outcomes_hit": int((counts > 0).sum().item()),
+            "cpu_baseline": cpu, "cpu_python_estimate": cpu_py}
+
+
+def _hist_cpu_lines(checker, host_samples, counts, n, m, S):
+    """CPU lines of the outcome histogram.
+    * cpu_baseline: the same rank-and-count on the host with numpy (each sample's index
+      in the reference's outcome order from a binomial table, then bincount), on the first
+      2^20 samples, checked against the device counts of those samples' ranks.
+    * cpu_python_estimate: the reference's loop (each sample compared with the outcomes
+      in order until one matches, bs_sample_based_distribution_calculator.py:92-106, as the
+      oracle restates it) costs a fixed time per comparison; that cost is timed on 200
+      samples of 4 photons in the same 20 modes and scaled by this run's mean scan length
+      (sum over outcomes of count x (index + 1))."""
+    comb = np.zeros((n + m + 1, m + 1), dtype=np.int64)
+    for a in range(n + m + 1):
+        for b in range(min(a, m) + 1):
+            comb[a, b] = math.comb(a, b)
+    x = host_samples.astype(np.int64)
+    t0 = time.perf_counter()
+    rank = np.zeros(len(x), dtype=np.int64)
+    rem = x.sum(axis=1)
+    for i in range(m - 1):
+        r = rem - x[:, i] - 1
+        ok = r >= 0
+        rank[ok] += comb[(r + m - i - 1)[ok], m - i - 1]
+        rem = rem - x[:, i]
+    host_counts = np.bincount(rank, minlength=math.comb(n + m - 1, n))
+    dt = time.perf_counter() - t0
+    cpu = {"value": len(x) / dt, "unit": "samples/s", "cores": 1, "kind": "port",
+           "sample": f"first {len(x)} samples: numpy rank from a binomial table and bincount "
+                     f"({dt:.2f} s)",
+           "distinct_ranks": int((host_counts > 0).sum()),
+           "ranks_counted_on_device": bool(np.all(counts.cpu().numpy()[host_counts > 0] > 0))}
+    small = checker.generate_possible_states(4, m)
+    rng = np.random.default_rng(5)
+    picks = rng.integers(0, len(small), 200)
+    probe = [small[i] for i in picks]
+    t0 = time.perf_counter()
+    checker.sample_frequencies(probe, small, len(probe))
+    per_cmp = (time.perf_counter() - t0) / float(np.sum(picks + 1))
+    c = counts.cpu().numpy().astype(np.float64)
+    mean_scan = float(np.sum(c * (np.arange(len(c)) + 1.0)) / max(c.sum(), 1.0))
+    est = S * mean_scan * per_cmp
+    cpu_py = {"value": S / est, "unit": "samples/s", "cores": 1, "kind": "port",
+              "est_seconds": est,
+              "sample": f"{per_cmp * 1e6:.2f} us per outcome comparison (200 samples of 4 photons "
+                        f"in {m} modes), times a mean scan of {mean_scan:.3g} outcomes per sample"}
+    return cpu, cpu_py
 
 
 def bobs(checker):
