@@ -295,20 +295,49 @@ def gcca(checker):
             "all_samples_have_n_photons": all(sum(s_) == n for s_ in samples)}
 
 
-def lossy(checker):
+def lossy_cpu(checker, U, n, eta, l0=18):
+    """CPU line of the uniform-loss sampler. A sample's cost is dominated by the binomial
+    tail of its photon number l (the k-th step sweeps 2^k terms), so a bounded sample
+    cannot show the mean: the oracle's C GCC-B sampler is timed at l = l0 (one sample per
+    thread) and scaled to every l by the C&C work sum_{k <= l} 2^k (10k - 12), then
+    averaged over Binomial(n, eta)."""
+    cores = _cpu_cores()
+    order = np.tile(np.arange(n, dtype=np.int32), (cores, 1))
+    unif = np.random.default_rng(1).random((cores, n))
+    steps = np.full(cores, l0, dtype=np.int32)
+    t0 = time.perf_counter()
+    checker.gccb_sample_batch(U, order, unif, steps, threads=cores)
+    t_l0 = time.perf_counter() - t0  # wall time of one sample per thread
+
+    def work(l):
+        return sum(2 ** k * (10 * k - 12) for k in range(2, l + 1))
+
+    mean = sum(math.comb(n, l) * eta ** l * (1 - eta) ** (n - l) * t_l0 * work(l) / work(l0)
+               for l in range(n + 1))
+    return {"value": cores / mean, "unit": "samples/s", "cores": cores, "kind": "port",
+            "est_seconds_per_sample_per_thread": mean,
+            "sample": f"oracle C GCC-B sampler, {cores} samples of l = {l0} photons on {cores} "
+                      f"threads ({t_l0:.2f} s), scaled by the C&C work to each l and averaged "
+                      f"over Binomial({n}, {eta})"}
+
+
+def lossy(checker, cpu=True):
     """Uniform-loss GCC-B, m=80, n=40 photons in modes 0..39, eta=0.5, 4096 samples."""
     T = D.torch()
     m, n, S, eta = 80, 40, 4096, 0.5
-    U = D.to_device_c128(checker.haar_unitary(m, 5))
+    Uh = checker.haar_unitary(m, 5)
+    U = D.to_device_c128(Uh)
     parts = D.to_device_i32(np.arange(n))
     out = T.empty((S, m), dtype=T.int32, device=U.device)
     ops.gccb_sample(U, parts, S, 1, eta, out=out)  # capture + warm
     T.cuda.synchronize()
     t = _time(lambda: ops.gccb_sample(U, parts, S, 2, eta, out=out), reps=2)
     l = out.sum(1).cpu().numpy()
+    cpu_rec = lossy_cpu(checker, Uh, n, eta) if cpu else None
     return {"workload": "uniform-loss GCC-B m=80 n=40 eta=0.5, 4096 samples",
             "samples_per_s": S / t, "ms_per_batch": t * 1e3, "mean_photons": float(l.mean()),
-            "max_photons": int(l.max()), "binomial_mean": n * eta}
+            "max_photons": int(l.max()), "binomial_mean": n * eta, "cpu_baseline": cpu_rec,
+            "gpu_vs_cpu_baseline": (S / t) / cpu_rec["value"] if cpu_rec else None}
 
 
 def fit(checker):
