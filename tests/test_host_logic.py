@@ -77,6 +77,29 @@ def test_clements_layout_counts():
         assert all(0 <= k < m - 1 for _, k in units)
 
 
+def test_clements_unitary_equals_product_of_unit_matrices():
+    """The column-batched mesh equals the product of the embedded 2x2 unit matrices
+    [[e^{i phi} cos theta, -sin theta], [e^{i phi} sin theta, cos theta]] applied on the
+    left in layout order, built with numpy."""
+    import torch
+
+    from theboss_amd.interferometer import clements_layout, clements_unitary
+
+    for m in (2, 5, 8):
+        nu = m * (m - 1) // 2
+        rng = np.random.default_rng(m)
+        th, ph = rng.random(nu) * 3, rng.random(nu) * 6
+        want = np.eye(m, dtype=np.complex128)
+        for u, (_, k) in enumerate(clements_layout(m)):
+            M = np.eye(m, dtype=np.complex128)
+            e = np.exp(1j * ph[u])
+            M[k, k], M[k, k + 1] = e * np.cos(th[u]), -np.sin(th[u])
+            M[k + 1, k], M[k + 1, k + 1] = e * np.sin(th[u]), np.cos(th[u])
+            want = M @ want
+        got = clements_unitary(torch.from_numpy(th), torch.from_numpy(ph), m).numpy()
+        assert np.max(np.abs(got - want)) < 1e-13, m
+
+
 def test_clements_unitary_cpu_is_unitary():
     import torch
 

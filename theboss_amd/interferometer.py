@@ -29,18 +29,27 @@ def clements_layout(m: int) -> List[Tuple[int, int]]:
 
 def clements_unitary(theta, phi, m: int):
     """Differentiable m x m unitary from the mesh parameters (torch tensors of length
-    m(m-1)/2, real)."""
+    m(m-1)/2, real): U = T_last ... T_1, unit u acting on rows (k, k+1). The units of one
+    column act on the disjoint row pairs (p, p+1), (p+2, p+3), ... (p = column parity), so
+    a column is one batched update of strided row views (about a dozen tensor operations
+    per column instead of per unit, and no index gathers whose backward must sort)."""
     T = D.torch()
     U = T.eye(m, dtype=T.complex128, device=theta.device)
-    rows = [U[i] for i in range(m)]
-    for u, (_, k) in enumerate(clements_layout(m)):
-        c = T.cos(theta[u]).to(T.complex128)
-        s = T.sin(theta[u]).to(T.complex128)
-        e = T.exp(1j * phi[u].to(T.complex128))
-        a, b = rows[k], rows[k + 1]
-        rows[k] = e * c * a - s * b
-        rows[k + 1] = e * s * a + c * b
-    return T.stack(rows)
+    u0 = 0
+    for col in range(m):
+        p = col % 2
+        L = len(range(p, m - 1, 2))  # units in this column
+        if L == 0:
+            continue
+        th, ph = theta[u0:u0 + L], phi[u0:u0 + L]
+        u0 += L
+        c = T.cos(th).to(T.complex128)[:, None]
+        s = T.sin(th).to(T.complex128)[:, None]
+        e = T.exp(1j * ph.to(T.complex128))[:, None]
+        a, b = U[p:p + 2 * L:2], U[p + 1:p + 2 * L:2]
+        mid = T.stack((e * c * a - s * b, e * s * a + c * b), dim=1).reshape(2 * L, m)
+        U = T.cat((U[:p], mid, U[p + 2 * L:]), dim=0)
+    return U
 
 
 def sample_index_tensors(samples: np.ndarray, input_modes: Sequence[int]):
