@@ -51,4 +51,4 @@ class BatchedGCCBSimulationStrategy(SimulationStrategyInterface):
 
     def simulate(self, input_state: Sequence[int], samples_number: int = 1) -> List[Tuple[int, ...]]:
         out, _ = self.sample_tensor(input_state, samples_number)
-        return [tuple(int(v) for v in row) for row in out.cpu().numpy()]
+        return list(map(tuple, out.cpu().tolist()))

@@ -41,13 +41,17 @@ def _particle_lists(inputs: np.ndarray, n_max: int) -> Tuple[np.ndarray, np.ndar
     """Mode assignment of each input row (sorted modes with multiplicity), padded with mode
     0 to n_max entries, and the particle counts: particle p of a row sits in the first
     mode whose running occupation exceeds p."""
-    counts = inputs.sum(axis=1).astype(np.int32)
+    inputs = np.asarray(inputs, dtype=np.int64)
+    S, m = inputs.shape
+    counts = inputs.sum(axis=1)
     width = max(n_max, 1)
-    cum = np.cumsum(inputs, axis=1)                                   # S x m
-    p = np.arange(width)
-    lists = (cum[:, :, None] <= p[None, None, :]).sum(axis=1)         # S x width
-    lists = np.where(p[None, :] < counts[:, None], lists, 0)
-    return lists.astype(np.int32), counts
+    # every particle's mode in mode order, then its position within its row
+    modes = np.repeat(np.tile(np.arange(m, dtype=np.int32), S), inputs.reshape(-1))
+    row = np.repeat(np.arange(S), counts)
+    pos = np.arange(len(modes)) - np.repeat(np.cumsum(counts) - counts, counts)
+    lists = np.zeros((S, width), dtype=np.int32)
+    lists[row, pos] = modes
+    return lists, counts.astype(np.int32)
 
 
 def expand_batch(Us: np.ndarray) -> np.ndarray:
@@ -176,7 +180,7 @@ class LossyStateApproximationSimulationStrategy(SimulationStrategyInterface):
         seed = (self._seed * 0x9E3779B97F4A7C15 + self._calls) & (2 ** 64 - 1)
         self._calls += 1
         out = _sample_on_device(Us, inputs, seed)
-        return [tuple(int(v) for v in row) for row in out]
+        return list(map(tuple, out.tolist()))
 
 
 class NonuniformLossesApproximationStrategy(SimulationStrategyInterface):
@@ -261,4 +265,4 @@ class NonuniformLossesApproximationStrategy(SimulationStrategyInterface):
         self._calls += 1
         padded = np.concatenate([inputs, np.zeros_like(inputs)], axis=1)
         out = _sample_on_device(expanded, padded, seed)
-        return [tuple(int(v) for v in row[:m]) for row in out]
+        return list(map(tuple, out[:, :m].tolist()))

@@ -473,7 +473,24 @@ def bobs(checker):
         one.simulate(init, 1)
     T.cuda.synchronize()
     t_one = (time.perf_counter() - t0) / 16
+    # CPU line: the reference simulates each sample as GCC-B on the 2m-mode dilation of its
+    # own interferometer with the photons left by the extracted uniform loss; the oracle's
+    # C sampler does the same work on a seeded 2m-mode unitary with photon numbers drawn
+    # from the same binomial, 16384 samples over the host threads.
+    cores = _cpu_cores()
+    rng = np.random.default_rng(9)
+    Sc = 16384
+    steps = rng.binomial(n, 1.0 - strat._uniform_losses, Sc).astype(np.int32)
+    order = np.stack([rng.permutation(n) for _ in range(Sc)]).astype(np.int32)
+    W = checker.haar_unitary(2 * m, 32)
+    t0 = time.perf_counter()
+    checker.gccb_sample_batch(W, order, rng.random((Sc, n)), steps, threads=cores)
+    t_cpu = time.perf_counter() - t0
+    cpu = {"value": Sc / t_cpu, "unit": "samples/s", "cores": cores, "kind": "port",
+           "sample": f"{Sc} GCC-B samples on a {2 * m}-mode unitary by the oracle's C sampler, "
+                     f"photons ~ Binomial({n}, {1.0 - strat._uniform_losses:.3f}) ({t_cpu:.2f} s)"}
     return {"workload": "BOBS non-uniform losses, m=16, n=8, 4 approximated modes, 4096 samples",
             "samples_per_s": S / t_batch, "ms_per_batch": t_batch * 1e3,
             "one_per_call_samples_per_s": 1.0 / t_one, "batched_vs_one_per_call": t_one * S / t_batch,
-            "mean_photons": float(np.mean([sum(s_) for s_ in samples]))}
+            "mean_photons": float(np.mean([sum(s_) for s_ in samples])),
+            "cpu_baseline": cpu, "gpu_vs_cpu_baseline": (S / t_batch) / cpu["value"]}
