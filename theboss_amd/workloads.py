@@ -258,11 +258,16 @@ def gcca(checker):
     m, n, S = 20, 10, 200
     U = checker.haar_unitary(m, 1)
     inp = [1] * n + [0] * (m - n)
+    # warm-up run of the full size with other draws: kernels loaded and device memory pools
+    # grown (the first 200-sample run in a process takes ~4x longer than later ones)
+    GeneralizedCliffordsSimulationStrategy(GPURyserPermanentCalculator(U),
+                                           rng=np.random.default_rng(1)).simulate(inp, S)
     strat = GeneralizedCliffordsSimulationStrategy(GPURyserPermanentCalculator(U),
                                                    rng=np.random.default_rng(0))
-    strat.simulate(inp, 2)  # warm: library and kernels loaded
+    D.torch().cuda.synchronize()
     t0 = time.perf_counter()
     samples = strat.simulate(inp, S)
+    D.torch().cuda.synchronize()
     dt = time.perf_counter() - t0
     ones = [1] * n + [0] * (m - n)
     r = GPURyserPermanentCalculator(U, ones, ones).compute_permanent()
