@@ -28,10 +28,17 @@ def _events():
     return T.cuda.Event(enable_timing=True), T.cuda.Event(enable_timing=True)
 
 
-def _time(fn, reps=1):
+def _time(fn, reps=1, warm_s=0.0):
+    """Mean seconds per call over reps calls, after one warm-up call, or after warm-up
+    calls filling warm_s seconds (a GPU idle before raises its clock over the first tens
+    of milliseconds of work, which short launches would otherwise be timed in)."""
     T = D.torch()
+    t_end = time.perf_counter() + warm_s
     fn()
     T.cuda.synchronize()
+    while time.perf_counter() < t_end:
+        fn()
+        T.cuda.synchronize()
     e0, e1 = _events()
     e0.record()
     for _ in range(reps):
@@ -84,28 +91,19 @@ def glynn(checker, ns=(24, 28, 32), precisions=("fp64", "fp32_colsum"), cpu=True
     a fixed time per Gray step and column."""
     with open(os.path.join(ROOT, "tests", "golden", "glynn_large.json")) as f:
         gold = json.load(f)["cases"]
-    rates = glynn_cpu_rates(checker) if cpu else None
     rows = []
     for n in ns:
         B = 16
         A = D.to_device_c128(_glynn_blocks(checker, n, B))
         fl = B * 8 * n * 2 ** (n - 1)
         rec = {"n": n, "batch": B, "flops": fl}
-        if rates:
-            c_rate, py_rate, cores, note_c, note_py = rates
-            work = B * 2 ** (n - 1) * n  # Gray steps x columns of the batch
-            wall_c = work * c_rate / min(cores, B)
-            wall_py = work * py_rate
-            rec["cpu_baseline"] = {"value": B / wall_c, "unit": "permanents/s", "cores": min(cores, B),
-                                   "kind": "port", "est_seconds": wall_c, "sample": note_c}
-            rec["cpu_python_estimate"] = {"value": B / wall_py, "unit": "permanents/s", "cores": 1,
-                                          "kind": "port", "est_seconds": wall_py, "sample": note_py}
         for prec, name in ((L.TB_PREC_FP64, "fp64"), (L.TB_PREC_FP32_COLSUM, "fp32_colsum")):
             if name not in precisions:
                 continue
             out = {}
             reps = 1 if n >= 30 else 5  # sub-10 ms launches: average a few
-            t = _time(lambda: out.setdefault("v", ops.glynn_batch(A, prec)), reps=reps)
+            t = _time(lambda: out.setdefault("v", ops.glynn_batch(A, prec)), reps=reps,
+                      warm_s=0.2)
             vals = out["v"].cpu().numpy()
             rec[name] = {"ms": t * 1e3, "tflops": fl / t / 1e12,
                          "frac_fp64_peak": fl / t / FP64_PEAK}
@@ -115,9 +113,23 @@ def glynn(checker, ns=(24, 28, 32), precisions=("fp64", "fp32_colsum"), cpu=True
             errs = [abs(vals[b] - ref[b]) / abs(ref[b]) for b in ref]
             rec[name]["max_rel_err_vs_oracle"] = max(errs)
             rec[name]["checked_against"] = f"{len(errs)} golden oracle values"
-            if rates:
-                rec[name]["gpu_vs_cpu_baseline"] = rec["cpu_baseline"]["est_seconds"] / t
         rows.append(rec)
+    # The CPU lines come after the GPU timings: a GPU left idle during seconds of host work
+    # starts the next short launches at a low clock.
+    if cpu:
+        c_rate, py_rate, cores, note_c, note_py = glynn_cpu_rates(checker)
+        for rec in rows:
+            n, B = rec["n"], rec["batch"]
+            work = B * 2 ** (n - 1) * n  # Gray steps x columns of the batch
+            wall_c = work * c_rate / min(cores, B)
+            wall_py = work * py_rate
+            rec["cpu_baseline"] = {"value": B / wall_c, "unit": "permanents/s", "cores": min(cores, B),
+                                   "kind": "port", "est_seconds": wall_c, "sample": note_c}
+            rec["cpu_python_estimate"] = {"value": B / wall_py, "unit": "permanents/s", "cores": 1,
+                                          "kind": "port", "est_seconds": wall_py, "sample": note_py}
+            for name in ("fp64", "fp32_colsum"):
+                if name in rec:
+                    rec[name]["gpu_vs_cpu_baseline"] = wall_c / (rec[name]["ms"] * 1e-3)
     return {"workload": "single large permanents (Glynn, 16 per n)", "results": rows}
 
 
@@ -162,7 +174,6 @@ def grad(checker, cpu=True, tune=True):
     for b in range(B):
         r = np.sort(np.random.default_rng(300 + b).choice(40, n, replace=False))
         mats.append(checker.haar_unitary(40, 300 + b)[r, :n])
-    cpu_rec = grad_cpu(checker, mats) if cpu else None
     tuned = None
     if tune:
         from . import autotune
@@ -182,6 +193,7 @@ def grad(checker, cpu=True, tune=True):
         permanent(Ag).backward(seed)
 
     t_pass = _time(fwd_bwd, reps=3)
+    cpu_rec = grad_cpu(checker, mats) if cpu else None  # after the GPU timings
     f_fwd = B * 8 * n * 2 ** (n - 1)
     # All n^2 minors by one sweep: 2^(n-1) (2n^2 + 20n) (n column adds, ~3n multiplies for
     # leave-one-out products, n^2 signed accumulations per step).
