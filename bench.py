@@ -76,7 +76,7 @@ def main() -> int:
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     if args.workload != "gccb":
-        from theboss_amd import workloads
+        import bench_workloads as workloads
 
         print(json.dumps(getattr(workloads, args.workload)(orc)))
         return 0
@@ -189,7 +189,7 @@ def main() -> int:
 
     # ---- the metric's second part: permanent + gradient (64 x n=20) and single large
     # permanents (16 x n=32) as % of the FP64 vector peak (full records under "workloads")
-    from theboss_amd import workloads
+    import bench_workloads as workloads
 
     wl_grad = workloads.grad(orc, cpu=not args.no_cpu_baseline)
     wl_glynn = workloads.glynn(orc, ns=(32,), precisions=("fp64",), cpu=not args.no_cpu_baseline)

@@ -1,5 +1,7 @@
-"""Measured runs of the secondary workloads (single large permanents, differentiation,
-GCC-A, uniform-loss GCC-B, interferometer fitting). Each returns one JSON-able record.
+"""bench.py's measured runs of the secondary workloads (single large permanents,
+differentiation, GCC-A, uniform-loss GCC-B, interferometer fitting, outcome histogram,
+BOBS and exact distributions). Each returns one JSON-able record. Kept beside bench.py,
+outside the theboss_amd package, because it drives the oracle.
 
 `checker` is the CPU oracle module passed in by the caller (bench.py); it produces
 seeded inputs and reference values, and its restatements of the reference's loops are
@@ -14,13 +16,13 @@ import time
 
 import numpy as np
 
-from . import _device as D
-from . import _lib as L
-from . import ops
+from theboss_amd import _device as D
+from theboss_amd import _lib as L
+from theboss_amd import ops
 
 FP64_PEAK = 78.6e12
 FP32_PEAK = 157.3e12
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ROOT = os.path.dirname(os.path.abspath(__file__))
 
 
 def _events():
@@ -176,10 +178,10 @@ def grad(checker, cpu=True, tune=True):
         mats.append(checker.haar_unitary(40, 300 + b)[r, :n])
     tuned = None
     if tune:
-        from . import autotune
+        from theboss_amd import autotune
 
         tuned = autotune.tune_grad([n], B, spread=4, reps=3)[n]
-    from .autograd import permanent
+    from theboss_amd.autograd import permanent
 
     T = D.torch()
     A = D.to_device_c128(np.stack(mats))
@@ -225,9 +227,9 @@ def exact(checker, m=12, n=6, eta=0.5):
     (outcome, lossy input) permanent by one batched Ryser launch per particle number.
     CPU line: the oracle's restatement of the reference's per-outcome loop
     (bs_exact_distribution_with_uniform_losses.py), C permanents, one core."""
-    from .distribution_calculators import (BosonSamplingExperimentConfiguration,
+    from theboss_amd.distribution_calculators import (BosonSamplingExperimentConfiguration,
                                            BSDistributionCalculatorWithUniformLosses)
-    from .permanent_calculators import GPURyserPermanentCalculator
+    from theboss_amd.permanent_calculators import GPURyserPermanentCalculator
 
     U = checker.haar_unitary(m, 1200 + m)
     inp = [1] * n + [0] * (m - n)
@@ -265,8 +267,8 @@ def gcca(checker):
     """GCC-A, m=20, n=10 photons in modes 0..9, 200 samples, Ryser on the GPU (the
     samples advance in lock-step: one all-pairs launch per particle number); plus one
     10x10 permanent by Ryser and Glynn."""
-    from .permanent_calculators import GPUGlynnGrayPermanentCalculator, GPURyserPermanentCalculator
-    from .simulation_strategies import GeneralizedCliffordsSimulationStrategy
+    from theboss_amd.permanent_calculators import GPUGlynnGrayPermanentCalculator, GPURyserPermanentCalculator
+    from theboss_amd.simulation_strategies import GeneralizedCliffordsSimulationStrategy
 
     m, n, S = 20, 10, 200
     U = checker.haar_unitary(m, 1)
@@ -361,7 +363,7 @@ def lossy(checker, cpu=True):
 def fit(checker):
     """Clements mesh m=16 (120 units) fitted by Adam to 10000 GPU samples (n=8) drawn
     from a seeded target unitary."""
-    from .interferometer import clements_unitary, fit_clements, negative_log_likelihood, sample_index_tensors
+    from theboss_amd.interferometer import clements_unitary, fit_clements, negative_log_likelihood, sample_index_tensors
 
     T = D.torch()
     m, n, S = 16, 8, 10000
@@ -467,8 +469,8 @@ def bobs(checker):
     """BOBS non-uniform-loss approximation: m=16 Haar modes with transmissivity 0.5..0.95
     by output mode, 8 photons in modes 0..7, 4 approximated modes, 4096 samples in one
     device call; against one sample per call (the reference's per-sample loop)."""
-    from .permanent_calculators import GPURyserPermanentCalculator
-    from .simulation_strategies import NonuniformLossesApproximationStrategy
+    from theboss_amd.permanent_calculators import GPURyserPermanentCalculator
+    from theboss_amd.simulation_strategies import NonuniformLossesApproximationStrategy
 
     T = D.torch()
     m, n, S, k = 16, 8, 4096, 4
