@@ -410,6 +410,22 @@ def test_batched_sampler_forced_draws_match_oracle(T, ops, oracle, m, n, S):
             hist.append(k)
 
 
+@pytest.mark.parametrize("m,n,S", [(5, 1, 65), (30, 12, 33), (30, 13, 100), (40, 14, 1),
+                                   (24, 2, 257)])
+def test_batched_sampler_edges_match_oracle(T, ops, oracle, m, n, S):
+    """Edges of the launch structure: one photon; n = 12 (every step in the fused launch);
+    n = 13, 14 (the fused steps then separate sweep and step launches); a single sample;
+    sample counts that are not multiples of the 64-sample init blocks."""
+    rng = np.random.default_rng(100 * n + S)
+    U = oracle.haar_unitary(m, 7 * m + n)
+    order, unif = _forced(rng, S, n)
+    out, _ = ops.gccb_sample(dev_c(T, U), dev_i(T, np.arange(n)), S, 1, 1.0, dev_i(T, order),
+                             T.from_numpy(unif).cuda(), None)
+    out = out.cpu().numpy()
+    assert np.array_equal(out, oracle.gccb_sample_batch(U, order, unif))
+    assert np.all(out.sum(axis=1) == n)
+
+
 def test_batched_sampler_uniform_loss_forced_steps(T, ops, oracle):
     m, n, S = 12, 6, 128
     rng = np.random.default_rng(1)
