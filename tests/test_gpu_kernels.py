@@ -343,11 +343,12 @@ def test_grad_single_sweep_vs_oracle_minors(T, ops, oracle, n, B):
         assert abs(perm[b] - p) <= REL * abs(p), (n, b)
 
 
-@pytest.mark.parametrize("n", [16, 18, 23, 24, 25, 26])
+@pytest.mark.parametrize("n", [16, 18, 23, 24, 25, 26, 27])
 def test_grad_single_sweep_laplace_large(T, ops, oracle, n):
     """Sizes outside the all-minors oracle test (n >= 23 is too slow for all n^2 oracle
     minors): Laplace expansion along every row and every column equals the Glynn
-    kernel's permanent, and three minors against the oracle."""
+    kernel's permanent, and three minors against the oracle. n = 27 is past the
+    single-sweep kernel (one C&C sweep per row instead)."""
     As = np.stack([oracle.haar_unitary(2 * n, 900 + n + b)[:n, :n] for b in range(2)])
     Ad = dev_c(T, As)
     perm, G = ops.perm_grad_batch(Ad)
