@@ -15,8 +15,10 @@
 // Samples are processed in lock-step per k. Uniform loss draws l ~ Binomial(n, eta) per
 // sample by CDF threshold (:67-85); samples are ordered by l (largest first), so the
 // active samples at step k are a prefix whose length is read from device memory by the
-// persistent sweep and step kernels. The whole sequence is captured once into a
-// hipGraph and replayed; per-call values (seed, eta) live in device memory.
+// persistent sweep and step kernels. Steps 1..12 of every sample run in one launch (one
+// wave per sample); each later step is a sweep launch and a step launch. The whole
+// sequence is captured once into a hipGraph and replayed; per-call values (seed, eta)
+// live in device memory.
 //
 // Randomness: Philox-4x32-10 (Salmon et al., SC'11) keyed by the seed, counter
 // (sample, step, purpose). Callers that need the reference's exact draws pass
