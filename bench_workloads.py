@@ -185,8 +185,10 @@ def grad(checker, cpu=True, tune=True):
 
     T = D.torch()
     A = D.to_device_c128(np.stack(mats))
-    t_fwd = _time(lambda: ops.glynn_batch(A), reps=3)  # forward alone (no grad)
-    t_bwd = _time(lambda: ops.perm_grad_batch(A), reps=3)
+    # 20 repetitions: the host work before the first launch after the start event is then
+    # a small share of the measured time (the passes themselves queue back to back)
+    t_fwd = _time(lambda: ops.glynn_batch(A), reps=20)  # forward alone (no grad)
+    t_bwd = _time(lambda: ops.perm_grad_batch(A), reps=20)
     Ag = A.clone().requires_grad_(True)
     seed = T.ones(B, dtype=T.complex128, device=A.device)
 
@@ -194,7 +196,7 @@ def grad(checker, cpu=True, tune=True):
         Ag.grad = None
         permanent(Ag).backward(seed)
 
-    t_pass = _time(fwd_bwd, reps=3)
+    t_pass = _time(fwd_bwd, reps=20)
     cpu_rec = grad_cpu(checker, mats) if cpu else None  # after the GPU timings
     f_fwd = B * 8 * n * 2 ** (n - 1)
     # All n^2 minors by one sweep: 2^(n-1) (2n^2 + 20n) (n column adds, ~3n multiplies for
