@@ -103,7 +103,7 @@ def glynn(checker, ns=(24, 28, 32), precisions=("fp64", "fp32_colsum"), cpu=True
             if name not in precisions:
                 continue
             out = {}
-            reps = 1 if n >= 30 else 5  # sub-10 ms launches: average a few
+            reps = 1 if n >= 30 else 20  # short launches: average many back to back
             t = _time(lambda: out.setdefault("v", ops.glynn_batch(A, prec)), reps=reps,
                       warm_s=0.2)
             vals = out["v"].cpu().numpy()
