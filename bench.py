@@ -102,7 +102,7 @@ def main() -> int:
         res, secs = autotune.timed(autotune.tune_sampler, M, N, SAMPLES)
         tuned = {"seconds": secs,
                  "chunk_bits": {k: r["chunk_bits"] for k, r in res.items()},
-                 "built_in": {k: autotune.sweep_builtin(k) for k in res}}
+                 "built_in": {k: autotune.sweep_builtin(k, SAMPLES) for k in res}}
 
     # ---- batched path: warmup then exactly K timed steps
     seeds = [SEED + i for i in range(args.warmup + args.steps)]
@@ -144,6 +144,10 @@ def main() -> int:
     torch.cuda.synchronize()
     t_launch = e0.elapsed_time(e1) / 1e3 / reps
     achieved = SAMPLES * sweep_flops(k) / t_launch / 1e12
+    import bench_workloads as workloads
+
+    _, sweep_ghz = workloads.clock_during(
+        lambda: [ops.sweep_minors_batch(Vt, k) for _ in range(reps)], 0.8 * reps * t_launch)
     traffic = None
     pmc = os.path.join(ROOT, "profiles", "sweep_k20_hbm_traffic.json")
     if os.path.exists(pmc):
@@ -189,8 +193,6 @@ def main() -> int:
 
     # ---- the metric's second part: permanent + gradient (64 x n=20) and single large
     # permanents (16 x n=32) as % of the FP64 vector peak (full records under "workloads")
-    import bench_workloads as workloads
-
     wl_grad = workloads.grad(orc, cpu=not args.no_cpu_baseline)
     wl_glynn = workloads.glynn(orc, ns=(32,), precisions=("fp64",), cpu=not args.no_cpu_baseline)
 
@@ -219,9 +221,12 @@ def main() -> int:
         "fp64_tflops_sampler": batched * sample_flops(N, M) / 1e12,
         "roofline": {
             "kernel": "sweep_kernel<20,3,3> (C&C minors sweep, k=20, 4096 items, chunk 2^%d)"
-                      % (tuned["chunk_bits"].get(N, autotune.sweep_builtin(N)) if tuned
-                         else autotune.sweep_builtin(N)),
+                      % (tuned["chunk_bits"].get(N, autotune.sweep_builtin(N, SAMPLES)) if tuned
+                         else autotune.sweep_builtin(N, SAMPLES)),
             "flops_per_launch": SAMPLES * sweep_flops(k),
+            "flop_count": "2^(k-1) (10k - 12) per item: the Glynn-type sweep's executed terms "
+                          "(a Ryser walk's 2^k (10k - 12) would double the numerator)",
+            "clock_ghz": sweep_ghz,
             "bound": "valu",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
@@ -233,8 +238,11 @@ def main() -> int:
         "cpu_baseline": cpu,
         "cpu_python_estimate": cpu_py,
         "autotune": tuned,
-        "perm_grad_frac_fp64_peak": wl_grad["frac_fp64_peak"],
+        "perm_grad_frac_fp64_peak_executed": wl_grad["frac_fp64_peak_executed"],
+        "perm_grad_frac_fp64_peak_effective": wl_grad["frac_fp64_peak_effective"],
         "glynn_n32_frac_fp64_peak": wl_glynn["results"][0]["fp64"]["frac_fp64_peak"],
+        "glynn_n32_clock_ghz": wl_glynn["results"][0]["fp64"]["clock_ghz"],
+        "glynn_n32_issue_bound_ceiling_frac": wl_glynn["results"][0]["fp64"]["issue_bound_ceiling_frac"],
         "workloads": {"perm_grad_64x20": wl_grad, "glynn_16x32": wl_glynn},
     }
     print(json.dumps(line))

@@ -50,6 +50,33 @@ def _time(fn, reps=1, warm_s=0.0):
     return e0.elapsed_time(e1) / 1e3 / reps
 
 
+def clock_during(fn, seconds):
+    """Run fn (which queues kernels on the current stream) with a one-wave clock probe on a
+    second stream that spans `seconds` of it; returns (fn's result, average shader clock in
+    GHz over that span, from the probe's shader-cycle and real-time counter deltas)."""
+    T = D.torch()
+    lib = L.load()
+    hz = int(lib.tb_realtime_hz())
+    L.check(min(hz, 0))
+    buf = T.zeros(2, dtype=T.int64, device="cuda")
+    side = T.cuda.Stream()  # default priority
+    T.cuda.synchronize()
+    res = fn()
+    with T.cuda.stream(side):
+        L.check(lib.tb_clock_probe(D.ptr(buf), max(1, int(seconds * hz)), D.stream_ptr()))
+    T.cuda.synchronize()
+    cycles, ticks = (int(v) for v in buf.cpu().tolist())
+    return res, cycles / ticks * hz / 1e9
+
+
+def glynn_issue_ceiling(n, clock_ghz):
+    """Fraction of the FP64 peak (2.4 GHz, every instruction an FMA) that the Glynn step can
+    reach when its instructions issue back to back: 8n counted flops per step against
+    2n column adds and n - 1 complex multiplies of 4 instructions (the last one fused into
+    the accumulation), i.e. 6n - 4 instructions of 2 peak flops each."""
+    return 8 * n / (2 * (6 * n - 4)) * clock_ghz / 2.4
+
+
 def _glynn_blocks(checker, n, B=16):
     mats = []
     for b in range(B):
@@ -99,7 +126,7 @@ def glynn(checker, ns=(24, 28, 32), precisions=("fp64", "fp32_colsum"), cpu=True
         A = D.to_device_c128(_glynn_blocks(checker, n, B))
         fl = B * 8 * n * 2 ** (n - 1)
         rec = {"n": n, "batch": B, "flops": fl}
-        for prec, name in ((L.TB_PREC_FP64, "fp64"), (L.TB_PREC_FP32_COLSUM, "fp32_colsum")):
+        for prec, name in ((L.TB_PREC_FP64, "fp64"), (L.TB_PREC_FP32_COLSUM_ANY_N, "fp32_colsum")):
             if name not in precisions:
                 continue
             out = {}
@@ -109,12 +136,17 @@ def glynn(checker, ns=(24, 28, 32), precisions=("fp64", "fp32_colsum"), cpu=True
             vals = out["v"].cpu().numpy()
             rec[name] = {"ms": t * 1e3, "tflops": fl / t / 1e12,
                          "frac_fp64_peak": fl / t / FP64_PEAK}
-            if prec == L.TB_PREC_FP32_COLSUM:
+            if prec == L.TB_PREC_FP32_COLSUM_ANY_N:
                 rec[name]["frac_fp32_peak"] = fl / t / FP32_PEAK
             ref = {c["b"]: complex(c["re"], c["im"]) for c in gold if c["n"] == n}
             errs = [abs(vals[b] - ref[b]) / abs(ref[b]) for b in ref]
             rec[name]["max_rel_err_vs_oracle"] = max(errs)
             rec[name]["checked_against"] = f"{len(errs)} golden oracle values"
+            if prec == L.TB_PREC_FP64:
+                # the same launch again with the clock probe spanning 80% of it
+                _, ghz = clock_during(lambda: ops.glynn_batch(A, prec), 0.8 * t)
+                rec[name]["clock_ghz"] = ghz
+                rec[name]["issue_bound_ceiling_frac"] = glynn_issue_ceiling(n, ghz)
         rows.append(rec)
     # The CPU lines come after the GPU timings: a GPU left idle during seconds of host work
     # starts the next short launches at a low clock.
@@ -197,6 +229,8 @@ def grad(checker, cpu=True, tune=True):
         permanent(Ag).backward(seed)
 
     t_pass = _time(fwd_bwd, reps=20)
+    # average shader clock over 20 back-to-back gradient launches
+    _, ghz = clock_during(lambda: [ops.perm_grad_batch(A) for _ in range(20)], 0.8 * 20 * t_bwd)
     cpu_rec = grad_cpu(checker, mats) if cpu else None  # after the GPU timings
     f_fwd = B * 8 * n * 2 ** (n - 1)
     # All n^2 minors by one sweep: 2^(n-1) (2n^2 + 20n) (n column adds, ~3n multiplies for
@@ -212,15 +246,21 @@ def grad(checker, cpu=True, tune=True):
             "cpu_baseline": cpu_rec,
             "autotune": tuned,
             "gpu_vs_cpu_baseline": (B / t_pass) / cpu_rec["value"] if cpu_rec else None,
-            "frac_fp64_peak": (f_fwd + f_grad) / t_pass / FP64_PEAK,
             "frac_fp64_peak_executed": f_bwd_exec / t_pass / FP64_PEAK,
+            "frac_fp64_peak_executed_kernels": f_bwd_exec / t_bwd / FP64_PEAK,
+            "frac_fp64_peak_effective": (f_fwd + f_grad) / t_pass / FP64_PEAK,
+            "clock_ghz": ghz,
             "perm_only_ms": t_fwd * 1e3, "perm_and_grad_kernel_ms": t_bwd * 1e3,
-            "flops_counted": f_fwd + f_grad, "flops_executed": f_bwd_exec,
+            "flops_executed": f_bwd_exec, "flops_effective": f_fwd + f_grad,
             "note": "autograd pass: the forward of permanent() on a tensor that requires grad "
                     "runs the gradient launch, which returns perm and all n^2 minors (one Gray "
                     "sweep per group of 8 columns); the backward scales the kept minors. "
-                    "Counted flops: Glynn forward 8n 2^(n-1) plus the single-sweep all-minors "
-                    "count 2^(n-1)(2n^2 + 20n), per matrix"}
+                    "executed: the flops the sweeps perform, ceil(n/8) 2^(n-1) (8n + 114) per "
+                    "matrix (the FLOP counter reads ~1.2x this: FMAs with a +-1 or +-2 operand "
+                    "count 2); _kernels: over the gradient launches alone, without the autograd "
+                    "host work. effective: algorithm-equivalent flops, a Glynn forward 8n 2^(n-1) "
+                    "plus the all-minors count 2^(n-1)(2n^2 + 20n) per matrix, which charges "
+                    "more than the single sweep executes (not a roofline fraction)"}
 
 
 def exact(checker, m=12, n=6, eta=0.5):

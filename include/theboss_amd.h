@@ -32,7 +32,12 @@ extern "C" {
 #define TB_EWORKSPACE (-4)  /* workspace too small */
 
 #define TB_PREC_FP64 0          /* FP64 column sums, products and accumulation */
-#define TB_PREC_FP32_COLSUM 1   /* FP32 column sums and products, FP64 accumulation */
+#define TB_PREC_FP32_COLSUM 1   /* FP32 column sums and products, FP64 accumulation, for
+                                   n <= TB_FP32_COLSUM_MAX_N; larger n run in FP64 */
+#define TB_PREC_FP32_COLSUM_ANY_N 2 /* the FP32 column-sum kernel at every n: for measuring
+                                       its error; above TB_FP32_COLSUM_MAX_N it misses the
+                                       1e-10 relative bar of the FP64 path */
+#define TB_FP32_COLSUM_MAX_N 23 /* largest n whose FP32 column sums stay in use */
 
 #define TB_FORMULA_RYSER 0
 #define TB_FORMULA_CHIN_HUH 1
@@ -72,17 +77,22 @@ int32_t tb_device_count(void);
  *   TB_TUNE_SWEEP  C&C minors sweep, size = k (also the steps of tb_gccb_sample)
  *   TB_TUNE_GLYNN  tb_glynn_batch, size = n
  *   TB_TUNE_GRAD   single-sweep gradient of tb_perm_grad_batch, size = n
- * chunk_bits 0 selects the built-in plan; other values are clamped to the kernel's
- * valid range when a call plans its launch. Process-wide and thread-safe; each call
- * reads an entry once, so its workspace check and its launch agree.
- * tb_tune_get returns the entry (>= 0) or a negative TB_E* code.
+ * Entries are kept per batch bucket: the batch of a launch (matrices, sweep items, or
+ * samples of tb_gccb_sample) falls in bucket ceil(log2(batch)), i.e. 1 | 2 | 3-4 | 5-8 |
+ * ... | above 2^(TB_TUNE_BATCH_BUCKETS - 2), because the best chunk for one permanent is
+ * not the best for thousands. tb_tune_set with batch <= 0 writes every bucket of the size.
+ * chunk_bits 0 selects the built-in plan (which also depends on the batch); other values
+ * are clamped to the kernel's valid range when a call plans its launch. Process-wide and
+ * thread-safe; each call reads an entry once, so its workspace check and its launch
+ * agree. tb_tune_get returns the entry (>= 0) or a negative TB_E* code.
  * ------------------------------------------------------------------------------- */
 #define TB_TUNE_SWEEP 1
 #define TB_TUNE_GLYNN 2
 #define TB_TUNE_GRAD 3
 #define TB_TUNE_MAX_SIZE 48
-int32_t tb_tune_set(int32_t kernel, int32_t size, int32_t chunk_bits);
-int32_t tb_tune_get(int32_t kernel, int32_t size);
+#define TB_TUNE_BATCH_BUCKETS 16
+int32_t tb_tune_set(int32_t kernel, int32_t size, int64_t batch, int32_t chunk_bits);
+int32_t tb_tune_get(int32_t kernel, int32_t size, int64_t batch);
 
 /* ---------------------------------------------------------------------------------
  * Glynn permanents of a batch of n x n matrices.
@@ -90,7 +100,10 @@ int32_t tb_tune_get(int32_t kernel, int32_t size);
  * (theboss/boson_sampling_utilities/permanent_calculators/glynn_gray_permanent_calculator.py:41-71)
  * applied to already-gathered effective matrices (boson_sampling_utilities.py:595-626).
  *   A          batch matrices, matrix b at A + b*matrix_stride (elements), row-major n x n
- *   precision  TB_PREC_FP64 or TB_PREC_FP32_COLSUM
+ *   precision  TB_PREC_FP64, TB_PREC_FP32_COLSUM or TB_PREC_FP32_COLSUM_ANY_N. The
+ *              reference computes in complex128 throughout; FP32 column sums lose
+ *              accuracy with n (relative error ~1e-5 at n = 24, ~3e-3 at n = 28..32), so
+ *              TB_PREC_FP32_COLSUM runs the FP64 kernel for n > TB_FP32_COLSUM_MAX_N.
  *   out        batch permanents
  * n == 0 yields 1 for every matrix (glynn_gray_permanent_calculator.py:52-53).
  * ------------------------------------------------------------------------------- */
@@ -158,7 +171,7 @@ int32_t tb_sweep_minors_batch(const tb_complex128 *Vt, int32_t k, int32_t ldv,
  * Permanent and its gradient d perm(A) / d A[i][j] = perm(A without row i, column j)
  * for a batch of n x n matrices. No reference counterpart (the reference has no
  * differentiation); row i of the gradient is what the C&C submatrices calculator returns
- * with row i deleted. For 14 <= n <= 26 one Gray sweep per group of 4 columns yields
+ * with row i deleted. For 14 <= n <= 26 one Gray sweep per group of 8 columns yields
  * all n rows of those columns (leave-one-out products and per-row correctors); other n
  * run one C&C minors sweep per deleted row. Both recompute instead of storing
  * per-subset sums.
@@ -181,18 +194,22 @@ int32_t tb_perm_grad_batch(const tb_complex128 *A, int32_t n, int32_t batch,
  * expansion over all m output modes and the inverse-CDF draw on the device.
  *
  *   U             m x m interferometer (row-major, ldu)
- *   particles     n input particles as mode indices (mode assignment of the input state,
- *                 boson_sampling_utilities.py:60-78)
+ *   particles     n input particles as mode indices in [0, m) (mode assignment of the
+ *                 input state, boson_sampling_utilities.py:60-78)
  *   seed          Philox-4x32-10 key for the device RNG
  *   samples       number of samples S
  *   transmissivity  eta in [0,1]; 1 = lossless GCC-B
- *   forced_order  optional S x n particle orders (NULL = device RNG)
+ *   forced_order  optional S x n particle orders, each row a permutation of 0..n-1
+ *                 (NULL = device RNG)
  *   forced_uniforms optional S x n uniforms for the output draws (NULL = device RNG)
  *   forced_steps  optional S step counts (NULL = n if eta == 1 else binomial draw)
  *   out_samples   S x m int32 output occupations
  *   trace_minors  optional S x n x n complex minors trace (step k row k-1), NULL = off
- * The sampler object owns no device memory: plan with tb_gccb_plan_size, pass a
+ * The sampler object owns no device memory: plan with tb_gccb_workspace_size, pass a
  * workspace of that many bytes.
+ * The index arrays are read on the device, after the call returns: a particles[] entry
+ * outside [0, m) or a forced_order[] entry outside [0, n) is clamped into range (no
+ * out-of-bounds read) and recorded in the workspace; tb_gccb_status then reports it.
  * ------------------------------------------------------------------------------- */
 typedef struct {
   int32_t m, n, samples;
@@ -212,6 +229,11 @@ int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128 *U, int32_
                        tb_complex128 *trace_minors, void *workspace, size_t workspace_bytes,
                        void *stream);
 
+/* Waits for `stream`, then returns TB_EINVAL (with a message) if the last tb_gccb_sample
+ * on this workspace clamped an out-of-range particles[] or forced_order[] entry, else
+ * TB_OK. No reference counterpart: the reference indexes Python lists, which raise. */
+int32_t tb_gccb_status(const void *workspace, void *stream);
+
 /* ---------------------------------------------------------------------------------
  * Histogram of output samples over the outcome list generate_possible_states(n, m,
  * losses) (boson_sampling_utilities.py:81-152): one block per particle number l
@@ -229,6 +251,18 @@ int64_t tb_outcome_count(int32_t n, int32_t m, int32_t losses);
 int32_t tb_outcome_histogram(const int32_t *samples, int32_t ld, int32_t samples_n, int32_t m,
                              int32_t n, int32_t losses, uint32_t *counts, uint32_t *outside,
                              void *stream);
+
+/* ---------------------------------------------------------------------------------
+ * Clock probe (measurement only; no reference counterpart). tb_clock_probe launches one
+ * wave on `stream` that records, into the device array out[2], the shader cycles and the
+ * real-time counter ticks that elapse while it waits for `ticks` real-time ticks (it also
+ * stops after a bounded number of waits). Launched on a second stream beside a long
+ * kernel, out[0] / out[1] * tb_realtime_hz() is the average shader clock during that
+ * kernel. tb_realtime_hz returns the real-time counter frequency of the current device
+ * (Hz) or a negative TB_E* code.
+ * ------------------------------------------------------------------------------- */
+int64_t tb_realtime_hz(void);
+int32_t tb_clock_probe(uint64_t *out, int64_t ticks, void *stream);
 
 #ifdef __cplusplus
 }

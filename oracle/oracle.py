@@ -559,14 +559,35 @@ def gcca_layer_pmf(U, input_state, r_sample, perm=perm_ryser) -> np.ndarray:
     return pmf
 
 
-def cc_submatrices_glynn(U, s, t) -> np.ndarray:
+def cc_submatrices_glynn(U, s, t, threads: int = 0) -> np.ndarray:
     """Oracle-grade minors for large k: each perm(U[t, s - e_i]) by Glynn with re-seeded
     column sums and double-double accumulation (mode 1). Independent of the Ryser walk,
-    whose alternating subset sums lose ~1e-10 relative accuracy by k = 20."""
+    whose alternating subset sums lose ~1e-10 relative accuracy by k = 20. The minors are
+    independent permanents, computed in parallel (threads 0 = all host threads)."""
     s = _ci(s)
     out = np.zeros(len(s), dtype=np.complex128)
-    for i in np.nonzero(s)[0]:
+    idx = np.nonzero(s)[0]
+    mats = []
+    for i in idx:
         sm = s.copy()
         sm[i] -= 1
-        out[i] = perm_glynn(U, sm, t, 1)
+        mats.append(effective_matrix(U, sm, t))
+    if not mats or mats[0].size == 0:  # one particle: the empty effective matrix gives 1
+        for i in idx:
+            out[i] = 1.0
+        return out
+    out[idx] = perm_glynn_batch(np.stack(mats), 1, threads)
+    return out
+
+
+def gcca_layer_pmf_batch(U, input_state, r_samples, threads: int = 0) -> np.ndarray:
+    """gcca_layer_pmf for many layers at once, by the C restatement
+    (orc_gcca_layer_pmf_batch, OpenMP over layers): rows of r_samples -> rows of pmfs."""
+    U = _cm(U)
+    m = U.shape[0]
+    inp = _ci(input_state)
+    rs = np.ascontiguousarray(np.asarray(r_samples, dtype=np.int32).reshape(-1, m))
+    out = np.zeros(rs.shape, dtype=np.float64)
+    lib().orc_gcca_layer_pmf_batch(_p(U), m, _p(inp), _p(rs), rs.shape[0], int(threads),
+                                   _p(out))
     return out

@@ -491,6 +491,79 @@ void orc_gccb_sample_batch(const cplx *U, int m, int n, int samples, const int *
   }
 }
 
+/* ---------------------------------------------------------------------------------
+ * GCC-A layer pmf (the unnormalised pmf of the next particle's output mode).
+ * theboss/simulation_strategies/generalized_cliffords_simulation_strategy.py:136-247:
+ * the next particle is number k = sum(r) + 1 (:141); the input substates s <= input with
+ * sum(s) = k (:94-121, :143-145) get weights w(s) = l! (n - l)! / n! prod_i C(in_i, in_i - s_i)
+ * with l = sum(in - s) (:190-205), normalised (:155); for each output r + e_o (:207-224)
+ *   pmf[o] = sum_s w(s) |perm(U[r + e_o, s])|^2 / (prod_i s_i! k!)        (:158-167, :226-247)
+ * with the permanent by the Ryser restatement above. Substates are enumerated as a
+ * mixed-radix counter over the modes (the reference's order differs; the sum is the same up
+ * to rounding). Layers: `layers` rows of r (m ints each); pmf: layers x m. OpenMP over layers.
+ * --------------------------------------------------------------------------------- */
+static double orc_fact(int v) {
+  double f = 1.0;
+  for (int i = 2; i <= v; ++i) f *= (double)i;
+  return f;
+}
+
+static double orc_binom(int a, int b) {
+  if (b < 0 || b > a) return 0.0;
+  return orc_fact(a) / (orc_fact(b) * orc_fact(a - b));
+}
+
+void orc_gcca_layer_pmf_batch(const cplx *U, int m, const int *input, const int *r_samples,
+                              int layers, int threads, double *pmf) {
+  int n = 0;
+  for (int i = 0; i < m; ++i) n += input[i];
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int L = 0; L < layers; ++L) {
+    const int *r = r_samples + (size_t)L * m;
+    double *out = pmf + (size_t)L * m;
+    int k = 1;
+    for (int i = 0; i < m; ++i) k += r[i];
+    int *s = (int *)calloc((size_t)m, sizeof(int));
+    int *t = (int *)malloc(sizeof(int) * (size_t)m);
+    /* pass 0: sum of the weights; pass 1: weighted probabilities */
+    double wsum = 0.0;
+    for (int o = 0; o < m; ++o) out[o] = 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int i = 0; i < m; ++i) s[i] = 0;
+      for (;;) {
+        int ks = 0;
+        for (int i = 0; i < m; ++i) ks += s[i];
+        if (ks == k) {
+          const int l = n - k;
+          double w = orc_fact(l) * orc_fact(n - l) / orc_fact(n);
+          for (int i = 0; i < m; ++i) w *= orc_binom(input[i], input[i] - s[i]);
+          if (pass == 0) {
+            wsum += w;
+          } else {
+            double norm = orc_fact(k);
+            for (int i = 0; i < m; ++i) norm *= orc_fact(s[i]);
+            for (int o = 0; o < m; ++o) {
+              memcpy(t, r, sizeof(int) * (size_t)m);
+              t[o] += 1;
+              const double a = cabs(orc_perm_guan(U, m, s, t, 0));
+              out[o] += (w / wsum) * (a * a / norm);
+            }
+          }
+        }
+        int i = 0;  /* next substate: mixed-radix increment, digit i in 0..input[i] */
+        while (i < m && s[i] == input[i]) s[i++] = 0;
+        if (i == m) break;
+        s[i] += 1;
+      }
+    }
+    free(t);
+    free(s);
+  }
+}
+
 int orc_omp_max_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

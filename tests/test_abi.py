@@ -84,25 +84,44 @@ def test_tuning_table_roundtrip_and_errors(lib):
 
     k = 20
     try:
-        assert lib.tb_tune_get(_lib.TB_TUNE_SWEEP, k) == 0  # built-in plan by default
+        assert lib.tb_tune_get(_lib.TB_TUNE_SWEEP, k, 4096) == 0  # built-in plan by default
         base = lib.tb_sweep_workspace_size(k, 4096)
-        assert lib.tb_tune_set(_lib.TB_TUNE_SWEEP, k, 8) == _lib.TB_OK
-        assert lib.tb_tune_get(_lib.TB_TUNE_SWEEP, k) == 8
+        assert lib.tb_tune_set(_lib.TB_TUNE_SWEEP, k, 4096, 8) == _lib.TB_OK
+        assert lib.tb_tune_get(_lib.TB_TUNE_SWEEP, k, 4096) == 8
+        # the bucket of 4096 (2049..4096) only: batch 4097 and batch 1 keep the built-in plan
+        assert lib.tb_tune_get(_lib.TB_TUNE_SWEEP, k, 2049) == 8
+        assert lib.tb_tune_get(_lib.TB_TUNE_SWEEP, k, 4097) == 0
+        assert lib.tb_tune_get(_lib.TB_TUNE_SWEEP, k, 1) == 0
         # 2^(19-8-6) waves per item instead of 2^(19-11-6): 8x the partial sums
         assert lib.tb_sweep_workspace_size(k, 4096) == 8 * base
-        assert lib.tb_tune_set(_lib.TB_TUNE_GLYNN, 32, 10) == _lib.TB_OK
-        assert lib.tb_tune_get(_lib.TB_TUNE_GLYNN, 32) == 10
-        assert lib.tb_tune_set(_lib.TB_TUNE_GRAD, 20, 9) == _lib.TB_OK
-        assert lib.tb_tune_get(_lib.TB_TUNE_GRAD, 20) == 9
+        assert lib.tb_tune_set(_lib.TB_TUNE_GLYNN, 32, 0, 10) == _lib.TB_OK  # every bucket
+        assert all(lib.tb_tune_get(_lib.TB_TUNE_GLYNN, 32, 1 << b) == 10
+                   for b in range(_lib.TB_TUNE_BATCH_BUCKETS))
+        assert lib.tb_tune_get(_lib.TB_TUNE_GLYNN, 32, 1 << 40) == 10  # last bucket
+        assert lib.tb_tune_set(_lib.TB_TUNE_GRAD, 20, 64, 9) == _lib.TB_OK
+        assert lib.tb_tune_get(_lib.TB_TUNE_GRAD, 20, 33) == 9
+        assert lib.tb_tune_get(_lib.TB_TUNE_GRAD, 20, 32) == 0
     finally:
         for kern, size in ((_lib.TB_TUNE_SWEEP, k), (_lib.TB_TUNE_GLYNN, 32), (_lib.TB_TUNE_GRAD, 20)):
-            lib.tb_tune_set(kern, size, 0)
+            lib.tb_tune_set(kern, size, 0, 0)
     assert lib.tb_sweep_workspace_size(k, 4096) == base
-    assert lib.tb_tune_set(7, 20, 8) == _lib.TB_EINVAL
-    assert lib.tb_tune_get(0, 20) == _lib.TB_EINVAL
-    assert lib.tb_tune_set(_lib.TB_TUNE_SWEEP, _lib.TB_TUNE_MAX_SIZE + 1, 8) == _lib.TB_ERANGE
-    assert lib.tb_tune_set(_lib.TB_TUNE_SWEEP, 20, 17) == _lib.TB_EINVAL
-    assert lib.tb_tune_set(_lib.TB_TUNE_SWEEP, 20, -1) == _lib.TB_EINVAL
+    assert lib.tb_tune_set(7, 20, 1, 8) == _lib.TB_EINVAL
+    assert lib.tb_tune_get(0, 20, 1) == _lib.TB_EINVAL
+    assert lib.tb_tune_get(_lib.TB_TUNE_SWEEP, 20, 0) == _lib.TB_EINVAL
+    assert lib.tb_tune_set(_lib.TB_TUNE_SWEEP, _lib.TB_TUNE_MAX_SIZE + 1, 1, 8) == _lib.TB_ERANGE
+    assert lib.tb_tune_set(_lib.TB_TUNE_SWEEP, 20, 1, 17) == _lib.TB_EINVAL
+    assert lib.tb_tune_set(_lib.TB_TUNE_SWEEP, 20, 1, -1) == _lib.TB_EINVAL
+
+
+def test_sweep_plan_fills_the_gpu_for_small_batches(lib):
+    """The built-in sweep plan depends on the batch: a single k = 20 item is cut into 2^5-step
+    chunks (256 waves: 2^14 chunks x 20 minors of partial sums), 4096 items keep 2^11-step
+    chunks (4 waves each)."""
+    k = 20
+    assert lib.tb_sweep_workspace_size(k, 1) == 256 * k * 16
+    assert lib.tb_sweep_workspace_size(k, 4096) == 4096 * 4 * k * 16
+    assert lib.tb_sweep_workspace_size(k, 16) == 16 * 256 * k * 16   # 4096 waves in total
+    assert lib.tb_sweep_workspace_size(k, 256) == 256 * 16 * k * 16  # 4096 waves in total
 
 
 def test_sampler_workspace_follows_tuned_sweeps(lib):
@@ -111,12 +130,29 @@ def test_sampler_workspace_follows_tuned_sweeps(lib):
     cfg = ops.gccb_config(400, 20, 4096)
     base = lib.tb_gccb_workspace_size(ctypes.byref(cfg))
     try:
-        lib.tb_tune_set(_lib.TB_TUNE_SWEEP, 20, 7)
+        lib.tb_tune_set(_lib.TB_TUNE_SWEEP, 20, 4096, 7)
         assert lib.tb_gccb_workspace_size(ctypes.byref(cfg)) > base
         cfg.chunk_bits[20] = 11  # an explicit per-call value wins over the table
         assert lib.tb_gccb_workspace_size(ctypes.byref(cfg)) == base
     finally:
-        lib.tb_tune_set(_lib.TB_TUNE_SWEEP, 20, 0)
+        lib.tb_tune_set(_lib.TB_TUNE_SWEEP, 20, 0, 0)
+
+
+def test_glynn_fp32_workspace_follows_the_fp64_rule(lib):
+    """FP32 column sums stop at n = 23: above, TB_PREC_FP32_COLSUM plans the FP64 launch
+    (no FP32 matrix copy in the workspace), TB_PREC_FP32_COLSUM_ANY_N the FP32 one."""
+    from theboss_amd import _lib
+
+    for n in (16, 23, 24, 32):
+        f64 = lib.tb_glynn_workspace_size(n, 16, _lib.TB_PREC_FP64)
+        f32 = lib.tb_glynn_workspace_size(n, 16, _lib.TB_PREC_FP32_COLSUM)
+        anyn = lib.tb_glynn_workspace_size(n, 16, _lib.TB_PREC_FP32_COLSUM_ANY_N)
+        copy = 16 * n * n * 8  # FP32 complex copy of the batch
+        assert anyn >= f64 + copy
+        if n <= _lib.TB_FP32_COLSUM_MAX_N:
+            assert f32 == anyn
+        else:
+            assert f32 == f64
 
 
 def test_outcome_count_host_only(lib):

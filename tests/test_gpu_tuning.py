@@ -46,7 +46,7 @@ def test_sweep_same_minors_at_every_chunk_size(T, oracle, tuning, k):
     Vd = dev_c(T, V)
     base = ops.sweep_minors_batch(Vd, k).cpu().numpy()
     for c in (3, 6, 9, 13):
-        tuning.set_entry("sweep", k, c)
+        tuning.set_entry("sweep", k, c, 64)
         got = ops.sweep_minors_batch(Vd, k).cpu().numpy()
         assert rel(got, base) <= REL, (k, c)
 
@@ -61,7 +61,7 @@ def test_glynn_same_permanents_at_every_chunk_size(T, oracle, tuning):
     # by ~1e-12 relative, so each chunk size is held to the oracle bar instead.
     ref = oracle.perm_glynn_batch(As, 1)
     for c in (0, 2, 5, 9, 12):
-        tuning.set_entry("glynn", n, c)
+        tuning.set_entry("glynn", n, c, 4)
         got = ops.glynn_batch(A).cpu().numpy()
         assert np.max(np.abs(got - ref) / np.abs(ref)) <= 1e-10, c
 
@@ -73,7 +73,7 @@ def test_grad_same_minors_at_every_chunk_size(T, oracle, tuning):
     A = dev_c(T, np.stack([oracle.haar_unitary(2 * n, 80 + b)[:n, :n] for b in range(8)]))
     p0, g0 = (x.cpu().numpy() for x in ops.perm_grad_batch(A))
     for c in (2, 3, 6, 9, 11):
-        tuning.set_entry("grad", n, c)
+        tuning.set_entry("grad", n, c, 8)
         p, g = (x.cpu().numpy() for x in ops.perm_grad_batch(A))
         assert rel(g, g0) <= REL, c
         # perm is the Laplace sum along row 0 of these minors, which cancels: its relative
@@ -94,7 +94,7 @@ def test_sampler_bit_exact_under_tuned_chunks(T, oracle, tuning):
     unif = T.from_numpy(rng.random((S, n))).cuda()
     base = ops.gccb_sample(U, parts, S, 1, forced_order=order, forced_uniforms=unif)[0].cpu().numpy()
     for k in range(8, n + 1):
-        tuning.set_entry("sweep", k, 4 + (k % 5))
+        tuning.set_entry("sweep", k, 4 + (k % 5), S)
     got = ops.gccb_sample(U, parts, S, 1, forced_order=order, forced_uniforms=unif)[0].cpu().numpy()
     assert np.array_equal(got, base)
 
@@ -104,10 +104,12 @@ def test_autotuner_fills_table_from_candidates(T, tuning):
     assert set(res) == {12, 16}
     for k, r in res.items():
         assert r["chunk_bits"] in r["seconds"]
-        assert tuning.get_entry("sweep", k) == r["chunk_bits"]
+        assert tuning.get_entry("sweep", k, 256) == r["chunk_bits"]
         fastest = min(r["seconds"].values())
         # the built-in choice stands unless a candidate beats it by the margin
         assert r["seconds"][r["chunk_bits"]] <= fastest / (1.0 - tuning.MARGIN)
     g = tuning.tune_grad([16], batch=8, reps=1)
-    assert tuning.get_entry("grad", 16) == g[16]["chunk_bits"]
-    assert tuning.table()["sweep"] == {k: r["chunk_bits"] for k, r in res.items()}
+    assert tuning.get_entry("grad", 16, 8) == g[16]["chunk_bits"]
+    bucket = tuning.batch_bucket(256)
+    assert tuning.table()["sweep"] == {f"{k}/{bucket}": r["chunk_bits"]
+                                       for k, r in res.items() if r["chunk_bits"]}

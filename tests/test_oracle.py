@@ -312,3 +312,110 @@ def test_total_variation_distance_closed_forms(oracle):
     assert oracle.total_variation_distance([0.2, 0.8], [0.2, 0.8]) == 0
     assert oracle.total_variation_distance([1, 0, 0], [0, 0.5, 0.5]) == 1
     assert abs(oracle.total_variation_distance([0.5, 0.5], [0.9, 0.1]) - 0.4) < 1e-15
+
+
+def test_gcca_layer_chain_reproduces_exact_output_distribution(oracle):
+    """Composing the GCC-A layers (normalised pmfs, one particle per layer) over every
+    sequence of picks gives the exact boson-sampling output distribution
+    |perm(U[t, s])|^2 / (prod s! prod t!), computed here by brute-force permutations: the
+    generalized C&C algorithm's defining property. Pins the layer weights and the
+    normalisation of gcca_layer_pmf (Python) and of its C restatement."""
+    import itertools
+
+    m = 4
+    U = oracle.haar_unitary(m, 41)
+    for s in ([1, 1, 1, 0], [2, 0, 1, 0]):
+        n = sum(s)
+        exact = {}
+        for t in oracle.generate_possible_states(n, m):
+            A = oracle.effective_matrix(U, s, t)
+            p = abs(oracle.perm_by_permutations(A)) ** 2
+            for v in list(s) + list(t):
+                p /= math.factorial(v)
+            exact[tuple(t)] = p
+        chain = {t: 0.0 for t in exact}
+        for picks in itertools.product(range(m), repeat=n):
+            r, prob = [0] * m, 1.0
+            for o in picks:
+                pmf = oracle.gcca_layer_pmf(U, s, r)
+                pmf_c = oracle.gcca_layer_pmf_batch(U, s, [r])[0]
+                assert np.allclose(pmf_c, pmf, rtol=1e-13, atol=1e-16)
+                prob *= pmf[o] / pmf.sum()
+                r[o] += 1
+            chain[tuple(r)] += prob
+        for t in exact:
+            assert abs(chain[t] - exact[t]) <= 1e-12, (s, t)
+
+
+def test_gcca_layer_pmf_c_equals_python_restatement(oracle):
+    """The C layer pmf (OpenMP over layers) equals the Python restatement on inputs with
+    mode multiplicities and on the m = 20, n = 10 workload's shapes."""
+    U = oracle.haar_unitary(6, 44)
+    inp = [2, 1, 0, 1, 1, 0]
+    rng = np.random.default_rng(4)
+    layers = []
+    for k in range(1, 6):
+        r = np.zeros(6, int)
+        for o in rng.integers(0, 6, k - 1):
+            r[o] += 1
+        layers.append(r)
+    got = oracle.gcca_layer_pmf_batch(U, inp, layers)
+    for r, g in zip(layers, got):
+        assert np.allclose(g, oracle.gcca_layer_pmf(U, inp, list(r)), rtol=1e-12, atol=1e-16)
+    U20 = oracle.haar_unitary(20, 1)
+    inp20 = [1] * 10 + [0] * 10
+    layers20 = [np.zeros(20, int), np.eye(20, dtype=int)[3] + np.eye(20, dtype=int)[17]]
+    got20 = oracle.gcca_layer_pmf_batch(U20, inp20, layers20)
+    for r, g in zip(layers20, got20):
+        assert np.allclose(g, oracle.gcca_layer_pmf(U20, inp20, list(r)), rtol=1e-12, atol=1e-17)
+
+
+def _gccb_reference_loop(oracle, U, order, uniforms, steps, perm):
+    """GCC-B with given randomness, composed here from the step formula of
+    generalized_cliffords_b_simulation_strategy.py:69-110: particle order[s][k] joins the
+    input, pmf[o] proportional to |perm(U[r + e_o, input])|^2, pick = first mode whose
+    running sum exceeds the uniform (numpy.random.choice)."""
+    m = U.shape[0]
+    out = np.zeros((len(order), m), dtype=np.int32)
+    for s in range(len(order)):
+        cur, r = np.zeros(m, int), np.zeros(m, int)
+        for k in range(int(steps[s])):
+            cur[order[s][k]] += 1
+            amp = np.array([abs(perm(U, cur, r + np.eye(m, dtype=int)[o])) ** 2 for o in range(m)])
+            cdf = np.cumsum(amp / amp.sum())
+            pick = int(np.searchsorted(cdf, uniforms[s][k], side="right"))
+            r[min(pick, m - 1)] += 1
+        out[s] = r
+    return out
+
+
+def test_gccb_sample_batch_equals_composed_step_formula(oracle):
+    """oracle.gccb_sample_batch (C, OpenMP over samples) against the step formula composed
+    in Python from brute-force classic permanents, with forced orders, uniforms and step
+    counts (0 and partial steps included)."""
+    m, n, S = 5, 3, 40
+    U = oracle.haar_unitary(m, 71)
+    rng = np.random.default_rng(71)
+    order = np.stack([rng.permutation(n) for _ in range(S)]).astype(np.int32)
+    unif = rng.random((S, n))
+    steps = rng.integers(0, n + 1, S).astype(np.int32)
+
+    def classic(U_, s_, t_):
+        return oracle.perm_by_permutations(oracle.effective_matrix(U_, s_, t_))
+
+    want = _gccb_reference_loop(oracle, U, order, unif, steps, classic)
+    assert np.array_equal(oracle.gccb_sample_batch(U, order, unif, steps), want)
+    full = _gccb_reference_loop(oracle, U, order, unif, np.full(S, n), classic)
+    assert np.array_equal(oracle.gccb_sample_batch(U, order, unif), full)
+    assert np.all(full.sum(axis=1) == n)
+
+
+def test_tvd_error_bound_hand_computed(oracle):
+    """quantum_computations_utilities.py:95-127: sqrt((-ln p + N ln 2) / (2 S)), checked
+    against values worked out by hand (ln 10 = 2.302585093, ln 2 = 0.693147181)."""
+    # N = 10, S = 1000, p = 1e-4: (9.210340372 + 6.931471806) / 2000 = 0.008070906089
+    assert abs(oracle.tvd_error_bound(10, 1000, 1e-4) - 0.0898382218) < 1e-9
+    # N = 1, S = 2, p = 0.5: (0.693147181 + 0.693147181) / 4 = 0.3465735903
+    assert abs(oracle.tvd_error_bound(1, 2, 0.5) - 0.5887050113) < 1e-9
+    # N = 0, S = 50, p = 1: 0
+    assert oracle.tvd_error_bound(0, 50, 1.0) == 0.0

@@ -18,6 +18,8 @@ TB_EWORKSPACE = -4
 
 TB_PREC_FP64 = 0
 TB_PREC_FP32_COLSUM = 1
+TB_PREC_FP32_COLSUM_ANY_N = 2
+TB_FP32_COLSUM_MAX_N = 23
 TB_FORMULA_RYSER = 0
 TB_FORMULA_CHIN_HUH = 1
 
@@ -25,6 +27,7 @@ TB_TUNE_SWEEP = 1
 TB_TUNE_GLYNN = 2
 TB_TUNE_GRAD = 3
 TB_TUNE_MAX_SIZE = 48
+TB_TUNE_BATCH_BUCKETS = 16
 
 TB_GLYNN_MAX_N = 48
 TB_SWEEP_MAX_K = 40
@@ -63,8 +66,8 @@ SIGNATURES = {
     "tb_version": (_i32, []),
     "tb_last_error": (ctypes.c_char_p, []),
     "tb_device_count": (_i32, []),
-    "tb_tune_set": (_i32, [_i32, _i32, _i32]),
-    "tb_tune_get": (_i32, [_i32, _i32]),
+    "tb_tune_set": (_i32, [_i32, _i32, _i64, _i32]),
+    "tb_tune_get": (_i32, [_i32, _i32, _i64]),
     "tb_glynn_workspace_size": (_sz, [_i32, _i32, _i32]),
     "tb_glynn_batch": (_i32, [_vp, _i32, _i32, _i64, _i32, _vp, _vp, _sz, _vp]),
     "tb_guan_perm_batch": (_i32, [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, _vp, _vp]),
@@ -82,6 +85,9 @@ SIGNATURES = {
         _i32,
         [ctypes.POINTER(GccbConfig), _vp, _i32, _vp, ctypes.c_uint64, ctypes.c_double, _vp, _vp,
          _vp, _vp, _vp, _vp, _sz, _vp]),
+    "tb_gccb_status": (_i32, [_vp, _vp]),
+    "tb_realtime_hz": (_i64, []),
+    "tb_clock_probe": (_i32, [_vp, _i64, _vp]),
 }
 
 _lock = threading.Lock()

@@ -11,7 +11,9 @@ and stores the fastest in the library's tuning table (tb_tune_set). Every later 
 that does not set a chunk size itself uses the table, including the batched sampler,
 whose captured graphs are keyed by the resolved chunk sizes.
 
-The table is process-wide; save()/load() keep it across processes.
+Entries are kept per (kernel, size, batch bucket): the chunk that serves one permanent
+best is not the one that serves thousands. The table is process-wide; save()/load()
+keep it across processes.
 """
 import json
 import time
@@ -26,29 +28,45 @@ from . import ops
 KERNELS = {"sweep": L.TB_TUNE_SWEEP, "glynn": L.TB_TUNE_GLYNN, "grad": L.TB_TUNE_GRAD}
 
 
-def set_entry(kernel: str, size: int, chunk_bits: int) -> None:
-    L.check(L.load().tb_tune_set(KERNELS[kernel], int(size), int(chunk_bits)))
+def batch_bucket(batch: int) -> int:
+    """The library's batch bucket of a launch: ceil(log2(batch)), capped at the last."""
+    b = 0
+    while b < L.TB_TUNE_BATCH_BUCKETS - 1 and (1 << b) < batch:
+        b += 1
+    return b
 
 
-def get_entry(kernel: str, size: int) -> int:
-    rc = L.load().tb_tune_get(KERNELS[kernel], int(size))
+def set_entry(kernel: str, size: int, chunk_bits: int, batch: int = 0) -> None:
+    """Tuning entry for `kernel` at `size` for launches of `batch` items (their bucket);
+    batch 0 sets every bucket of the size."""
+    L.check(L.load().tb_tune_set(KERNELS[kernel], int(size), int(batch), int(chunk_bits)))
+
+
+def get_entry(kernel: str, size: int, batch: int) -> int:
+    rc = L.load().tb_tune_get(KERNELS[kernel], int(size), int(batch))
     L.check(min(rc, 0))
     return rc
 
 
 def reset() -> None:
-    """Back to the built-in plans for every kernel and size."""
+    """Back to the built-in plans for every kernel, size and batch."""
     for name in KERNELS:
         for size in range(L.TB_TUNE_MAX_SIZE + 1):
             set_entry(name, size, 0)
 
 
-def table() -> Dict[str, Dict[int, int]]:
-    """Non-zero entries of the tuning table."""
-    out: Dict[str, Dict[int, int]] = {}
+def table() -> Dict[str, Dict[str, int]]:
+    """Non-zero entries of the tuning table: {kernel: {"size/bucket": chunk_bits}}, where
+    bucket b holds batches in (2^(b-1), 2^b]."""
+    out: Dict[str, Dict[str, int]] = {}
     for name in KERNELS:
-        ent = {s: get_entry(name, s) for s in range(L.TB_TUNE_MAX_SIZE + 1)}
-        out[name] = {s: c for s, c in ent.items() if c}
+        ent = {}
+        for s in range(L.TB_TUNE_MAX_SIZE + 1):
+            for b in range(L.TB_TUNE_BATCH_BUCKETS):
+                c = get_entry(name, s, 1 << b)
+                if c:
+                    ent[f"{s}/{b}"] = c
+        out[name] = ent
     return out
 
 
@@ -61,15 +79,20 @@ def load(path: str) -> None:
     with open(path) as f:
         data = json.load(f)
     for name, ent in data["table"].items():
-        for size, c in ent.items():
-            set_entry(name, int(size), int(c))
+        for key, c in ent.items():
+            size, bucket = (int(v) for v in key.split("/"))
+            set_entry(name, size, int(c), 1 << bucket)
 
 
 # ------------------------------------------------------------------ built-in plans
 # Mirrors of the library's built-in chunk choices, used only to centre the candidates.
-def sweep_builtin(k: int) -> int:
+def sweep_builtin(k: int, items: int) -> int:
     nb = k - 1
-    return max(min(11, nb - 6), min(3, nb))
+    want = 0
+    while (max(items, 1) << want) < 4096 and want < 30:
+        want += 1
+    c = max(min(11, nb - 6 - want), min(5, nb - 6))
+    return min(max(c, min(3, nb)), nb, 16)
 
 
 def glynn_builtin(n: int, batch: int) -> int:
@@ -122,15 +145,16 @@ def _time(fn, reps: int) -> float:
 MARGIN = 0.03
 
 
-def _pick(kernel: str, size: int, cands: Iterable[int], builtin: int, fn, reps: int) -> dict:
+def _pick(kernel: str, size: int, batch: int, cands: Iterable[int], builtin: int, fn,
+          reps: int) -> dict:
     times = {}
     for c in cands:
-        set_entry(kernel, size, c)
+        set_entry(kernel, size, c, batch)
         times[c] = _time(fn, reps)
     best = min(times, key=times.get)
     if builtin in times and times[best] > (1.0 - MARGIN) * times[builtin]:
         best = builtin
-    set_entry(kernel, size, best)
+    set_entry(kernel, size, best, batch)
     return {"chunk_bits": best, "seconds": {c: t for c, t in sorted(times.items())}}
 
 
@@ -142,20 +166,19 @@ def _haar_block(rows: int, cols: int, seed: int) -> np.ndarray:
 
 
 def tune_sweep(ks: Iterable[int], items: int = 4096, spread: int = 2, reps: int = 3,
-               seed: int = 0) -> Dict[int, dict]:
-    """C&C minors sweep at k particles over `items` permanents (fewer for k > 20 so a
-    launch stays short). Returns {k: {"chunk_bits", "seconds"}}."""
+               seed: int = 0, min_k: int = 8) -> Dict[int, dict]:
+    """C&C minors sweep at k particles over `items` permanents (the entry of that batch's
+    bucket). Returns {k: {"chunk_bits", "seconds", "items"}}."""
     out = {}
     for k in ks:
-        if k < 8 or k > L.TB_SWEEP_MAX_K:
+        if k < min_k or k > L.TB_SWEEP_MAX_K:
             continue  # sub-millisecond launches: the built-in plan stands
-        n_items = max(64, items >> max(0, k - 20))
         V = _haar_block(k, k, seed + k)[: k - 1, :].T.copy()  # particle-major k x (k-1)
-        Vd = D.to_device_c128(np.ascontiguousarray(np.broadcast_to(V, (n_items, k, k - 1))))
-        c0 = sweep_builtin(k)
+        Vd = D.to_device_c128(np.ascontiguousarray(np.broadcast_to(V, (items, k, k - 1))))
+        c0 = sweep_builtin(k, items)
         cands = _candidates(c0, 3, min(k - 1, 16), spread)
-        out[k] = _pick("sweep", k, cands, c0, lambda: ops.sweep_minors_batch(Vd, k), reps)
-        out[k]["items"] = n_items
+        out[k] = _pick("sweep", k, items, cands, c0, lambda: ops.sweep_minors_batch(Vd, k), reps)
+        out[k]["items"] = items
     return out
 
 
@@ -176,7 +199,7 @@ def tune_glynn(ns: Iterable[int], batch: int, spread: int = 2, reps: int = 2,
                                        for b in range(batch)]))
         c0 = glynn_builtin(n, batch)
         cands = _candidates(c0, 2, min(12, n - 9), spread)
-        out[n] = _pick("glynn", n, cands, c0, lambda: ops.glynn_batch(A), reps)
+        out[n] = _pick("glynn", n, batch, cands, c0, lambda: ops.glynn_batch(A), reps)
     return out
 
 
@@ -192,7 +215,7 @@ def tune_grad(ns: Iterable[int], batch: int, spread: int = 2, reps: int = 3,
         u = grad_static_bits(n)
         c0 = grad_builtin(n, batch)
         cands = _candidates(c0, u, min(11, n - 8), spread)
-        out[n] = _pick("grad", n, cands, c0, lambda: ops.perm_grad_batch(A), reps)
+        out[n] = _pick("grad", n, batch, cands, c0, lambda: ops.perm_grad_batch(A), reps)
     return out
 
 

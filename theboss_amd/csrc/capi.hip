@@ -10,7 +10,7 @@
 namespace {
 thread_local char g_last_error[512] = "";
 // Tuning table, zero (built-in plan) until set; static storage is zero-initialised.
-std::atomic<int32_t> g_tune[TB_TUNE_GRAD + 1][TB_TUNE_MAX_SIZE + 1];
+std::atomic<int32_t> g_tune[TB_TUNE_GRAD + 1][TB_TUNE_MAX_SIZE + 1][TB_TUNE_BATCH_BUCKETS];
 }
 
 namespace tbh {
@@ -28,31 +28,50 @@ int32_t check_hip(hipError_t e, const char *what) {
   return fail(TB_EHIP, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
 }
 
-int32_t tuned(int kernel, int size) {
+// Bucket ceil(log2(batch)): 1 | 2 | 3-4 | 5-8 | ... | above 2^(buckets-2).
+int tune_bucket(int64_t batch) {
+  int b = 0;
+  while (b < TB_TUNE_BATCH_BUCKETS - 1 && ((int64_t)1 << b) < batch) ++b;
+  return b;
+}
+
+int32_t tuned(int kernel, int size, int64_t batch) {
   if (kernel < TB_TUNE_SWEEP || kernel > TB_TUNE_GRAD || size < 0 || size > TB_TUNE_MAX_SIZE)
     return 0;
-  return g_tune[kernel][size].load(std::memory_order_relaxed);
+  return g_tune[kernel][size][tune_bucket(batch)].load(std::memory_order_relaxed);
 }
 
 }  // namespace tbh
 
-extern "C" int32_t tb_tune_set(int32_t kernel, int32_t size, int32_t chunk_bits) {
+namespace {
+int32_t tune_args(int32_t kernel, int32_t size) {
   if (kernel < TB_TUNE_SWEEP || kernel > TB_TUNE_GRAD)
     return tbh::fail(TB_EINVAL, "tune: unknown kernel %d", kernel);
   if (size < 0 || size > TB_TUNE_MAX_SIZE)
     return tbh::fail(TB_ERANGE, "tune: size %d outside 0..%d", size, TB_TUNE_MAX_SIZE);
+  return TB_OK;
+}
+}  // namespace
+
+extern "C" int32_t tb_tune_set(int32_t kernel, int32_t size, int64_t batch, int32_t chunk_bits) {
+  const int32_t rc = tune_args(kernel, size);
+  if (rc) return rc;
   if (chunk_bits < 0 || chunk_bits > 16)
     return tbh::fail(TB_EINVAL, "tune: chunk_bits %d outside 0..16", chunk_bits);
-  g_tune[kernel][size].store(chunk_bits, std::memory_order_relaxed);
+  if (batch <= 0) {  // every bucket
+    for (int b = 0; b < TB_TUNE_BATCH_BUCKETS; ++b)
+      g_tune[kernel][size][b].store(chunk_bits, std::memory_order_relaxed);
+  } else {
+    g_tune[kernel][size][tbh::tune_bucket(batch)].store(chunk_bits, std::memory_order_relaxed);
+  }
   return TB_OK;
 }
 
-extern "C" int32_t tb_tune_get(int32_t kernel, int32_t size) {
-  if (kernel < TB_TUNE_SWEEP || kernel > TB_TUNE_GRAD)
-    return tbh::fail(TB_EINVAL, "tune: unknown kernel %d", kernel);
-  if (size < 0 || size > TB_TUNE_MAX_SIZE)
-    return tbh::fail(TB_ERANGE, "tune: size %d outside 0..%d", size, TB_TUNE_MAX_SIZE);
-  return g_tune[kernel][size].load(std::memory_order_relaxed);
+extern "C" int32_t tb_tune_get(int32_t kernel, int32_t size, int64_t batch) {
+  const int32_t rc = tune_args(kernel, size);
+  if (rc) return rc;
+  if (batch < 1) return tbh::fail(TB_EINVAL, "tune: batch %lld < 1", (long long)batch);
+  return g_tune[kernel][size][tbh::tune_bucket(batch)].load(std::memory_order_relaxed);
 }
 
 extern "C" int32_t tb_version(void) { return 100; /* 0.1.0 */ }

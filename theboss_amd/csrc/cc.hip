@@ -141,7 +141,7 @@ extern "C" size_t tb_cc_workspace_size(int32_t k, int32_t out_max, int32_t in_ma
   (void)out_max;
   (void)in_max;
   if (k < 2 || k > TB_SWEEP_MAX_K) return 0;
-  return cc_workspace_bytes(k, batch, tbh::tuned(TB_TUNE_SWEEP, k));
+  return cc_workspace_bytes(k, batch, tbh::tuned(TB_TUNE_SWEEP, k, batch));
 }
 
 extern "C" int32_t tb_cc_submatrices_batch(const tb_complex128 *U, int32_t ldu,
@@ -175,7 +175,7 @@ extern "C" int32_t tb_cc_submatrices_batch(const tb_complex128 *U, int32_t ldu,
   }
   if (k > TB_SWEEP_MAX_K)
     return tbh::fail(TB_ERANGE, "cc: k=%d exceeds TB_SWEEP_MAX_K=%d", k, TB_SWEEP_MAX_K);
-  const int c = tbh::tuned(TB_TUNE_SWEEP, k);  // read once: size check and launch agree
+  const int c = tbh::tuned(TB_TUNE_SWEEP, k, batch);  // read once: size check and launch agree
   const size_t need = cc_workspace_bytes(k, batch, c);
   if (!workspace || workspace_bytes < need)
     return tbh::fail(TB_EWORKSPACE, "cc: workspace %zu bytes < required %zu", workspace_bytes,
@@ -202,9 +202,11 @@ extern "C" int32_t tb_cc_submatrices_batch(const tb_complex128 *U, int32_t ldu,
 }
 
 namespace {
-// c: the tuning entry of the path that n takes (single-sweep gradient or n sweeps).
-int grad_tuned(int n) {
-  return tbh::tuned(tbgr::grad_supported(n) ? TB_TUNE_GRAD : TB_TUNE_SWEEP, n);
+// c: the tuning entry of the path that n takes (single-sweep gradient over the batch, or
+// n sweeps per matrix over batch * n sweep items).
+int grad_tuned(int n, int batch) {
+  if (tbgr::grad_supported(n)) return tbh::tuned(TB_TUNE_GRAD, n, batch);
+  return tbh::tuned(TB_TUNE_SWEEP, n, (int64_t)batch * n);
 }
 
 size_t grad_workspace_bytes(int n, int batch, int c) {
@@ -218,7 +220,7 @@ size_t grad_workspace_bytes(int n, int batch, int c) {
 
 extern "C" size_t tb_perm_grad_workspace_size(int32_t n, int32_t batch) {
   if (n < 2 || n > TB_SWEEP_MAX_K) return 0;
-  return grad_workspace_bytes(n, batch, grad_tuned(n));
+  return grad_workspace_bytes(n, batch, grad_tuned(n, batch));
 }
 
 extern "C" int32_t tb_perm_grad_batch(const tb_complex128 *A, int32_t n, int32_t batch,
@@ -240,7 +242,7 @@ extern "C" int32_t tb_perm_grad_batch(const tb_complex128 *A, int32_t n, int32_t
                        (tb::cd *)grad_out);
     return tbh::check_hip(hipGetLastError(), "grad_small_kernel");
   }
-  const int c = grad_tuned(n);  // read once: size check and launch agree
+  const int c = grad_tuned(n, batch);  // read once: size check and launch agree
   const size_t need = grad_workspace_bytes(n, batch, c);
   if (!workspace || workspace_bytes < need)
     return tbh::fail(TB_EWORKSPACE, "grad: workspace %zu bytes < required %zu", workspace_bytes,

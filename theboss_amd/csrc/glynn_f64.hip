@@ -20,16 +20,23 @@ size_t tbgl_partial_bytes(int n, int batch, int c_pref) {
   return align256((size_t)batch * wpm * sizeof(tb::cd));
 }
 
+// The precision a call runs in: FP32 column sums only up to TB_FP32_COLSUM_MAX_N unless
+// the caller asks for them at any n (error studies); larger n keep FP64 end to end.
+bool runs_fp32(int n, int precision) {
+  return precision == TB_PREC_FP32_COLSUM_ANY_N ||
+         (precision == TB_PREC_FP32_COLSUM && n <= TB_FP32_COLSUM_MAX_N);
+}
+
 size_t glynn_workspace_bytes(int n, int batch, int precision, int c_pref) {
   if (n <= 1 || batch <= 0 || n > TB_GLYNN_MAX_N) return 0;
   size_t bytes = tbgl_partial_bytes(n, batch, c_pref);
-  if (precision == TB_PREC_FP32_COLSUM) bytes += align256((size_t)batch * n * n * sizeof(tb::cpx<float>));
+  if (runs_fp32(n, precision)) bytes += align256((size_t)batch * n * n * sizeof(tb::cpx<float>));
   return bytes;
 }
 }  // namespace
 
 extern "C" size_t tb_glynn_workspace_size(int32_t n, int32_t batch, int32_t precision) {
-  return glynn_workspace_bytes(n, batch, precision, tbh::tuned(TB_TUNE_GLYNN, n));
+  return glynn_workspace_bytes(n, batch, precision, tbh::tuned(TB_TUNE_GLYNN, n, batch));
 }
 
 extern "C" int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t batch,
@@ -42,7 +49,8 @@ extern "C" int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t bat
   if (n < 0) return tbh::fail(TB_EINVAL, "glynn: n=%d < 0", n);
   if (n > TB_GLYNN_MAX_N)
     return tbh::fail(TB_ERANGE, "glynn: n=%d exceeds TB_GLYNN_MAX_N=%d", n, TB_GLYNN_MAX_N);
-  if (precision != TB_PREC_FP64 && precision != TB_PREC_FP32_COLSUM)
+  if (precision != TB_PREC_FP64 && precision != TB_PREC_FP32_COLSUM &&
+      precision != TB_PREC_FP32_COLSUM_ANY_N)
     return tbh::fail(TB_EINVAL, "glynn: unknown precision %d", precision);
   tb::cd *o = (tb::cd *)out;
   if (n == 0) {
@@ -53,7 +61,7 @@ extern "C" int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t bat
   if (matrix_stride < (int64_t)n * n)
     return tbh::fail(TB_EINVAL, "glynn: matrix_stride=%lld < n*n=%d", (long long)matrix_stride,
                      n * n);
-  const int c_pref = tbh::tuned(TB_TUNE_GLYNN, n);  // read once: size check and launch agree
+  const int c_pref = tbh::tuned(TB_TUNE_GLYNN, n, batch);  // read once: size check and launch agree
   const size_t need = glynn_workspace_bytes(n, batch, precision, c_pref);
   if (need > 0 && (workspace == nullptr || workspace_bytes < need))
     return tbh::fail(TB_EWORKSPACE, "glynn: workspace %zu bytes < required %zu", workspace_bytes,
@@ -65,7 +73,7 @@ extern "C" int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t bat
   }
   char *ws = (char *)workspace;
   tb::cd *partial = (tb::cd *)ws;
-  if (precision == TB_PREC_FP64)
+  if (!runs_fp32(n, precision))
     return tbgl::glynn_dispatch_f64(n, (const tb::cpx<double> *)A, matrix_stride, batch, o,
                                     partial, c_pref, s);
   // FP32 column sums: convert once into the tail of the workspace.

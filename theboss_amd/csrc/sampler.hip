@@ -23,6 +23,8 @@
 // Randomness: Philox-4x32-10 (Salmon et al., SC'11) keyed by the seed, counter
 // (sample, step, purpose). Callers that need the reference's exact draws pass
 // forced orders / uniforms / step counts instead.
+#include <stddef.h>
+
 #include <algorithm>
 #include <map>
 #include <mutex>
@@ -36,7 +38,7 @@ size_t sweep_partial_bytes(int k, int items, int c_override);
 int32_t run_sweep_partial(int k, const tb::cd *Vt, int ldv, int64_t item_stride,
                           const int *item_map, const int *item_count, int items, int c_override,
                           tb::cd *partial, int *wpi, hipStream_t s);
-void sweep_plan_of(int k, int c_override, int *c, int *wpi, int *iters);
+void sweep_plan_of(int k, int items, int c_override, int *c, int *wpi, int *iters);
 }  // namespace tbs
 
 namespace tb {
@@ -86,7 +88,11 @@ struct GccbParams {
   int64_t u_stride;  // per-sample interferometers: sample s uses U + s * u_stride
   int ldu;
   int per_sample;    // 1: per-sample U and particle lists (particles + s * n)
+  int clamped;       // set by the kernels when an index argument was out of range
 };
+
+// Bits of GccbParams::clamped.
+enum : int { kClampParticle = 1, kClampOrder = 2 };
 
 __global__ void gccb_set_params_kernel(GccbParams *p, GccbParams v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
@@ -124,7 +130,18 @@ __global__ __launch_bounds__(kInitBlock) void gccb_init_kernel(
     // that prefix is shuffled, so the padding beyond it never enters a step.
     const int span = (prm->per_sample && forced_steps) ? min(max(forced_steps[s], 0), n) : n;
     if (forced_order) {
-      for (int i = 0; i < n; ++i) ord[i * kInitBlock] = forced_order[(size_t)s * n + i];
+      // Entries outside [0, n) would index past the sample's particle rows: clamp them
+      // (the call then reports TB_EINVAL through tb_gccb_status).
+      bool bad = false;
+      for (int i = 0; i < n; ++i) {
+        int v = forced_order[(size_t)s * n + i];
+        if (v < 0 || v >= n) {
+          bad = true;
+          v = v < 0 ? 0 : n - 1;
+        }
+        ord[i * kInitBlock] = v;
+      }
+      if (bad) atomicOr(const_cast<int *>(&prm->clamped), kClampOrder);
     } else {
       for (int i = 0; i < n; ++i) ord[i * kInitBlock] = i;
       for (int i = span - 1; i >= 1; --i) {  // Fisher-Yates
@@ -228,7 +245,12 @@ __global__ void gccb_uin_kernel(const GccbParams *__restrict__ prm, int m, int n
   const int rem = (int)(idx - (int64_t)s * m * n);
   const int nu = rem / m, mm = rem - nu * m;
   const cd *U = prm->U + (size_t)s * prm->u_stride;
-  UinT[idx] = U[(size_t)mm * prm->ldu + prm->particles[(size_t)s * n + nu]];
+  int col = prm->particles[(size_t)s * n + nu];
+  if (col < 0 || col >= m) {  // not a mode index: clamp, and report through tb_gccb_status
+    if (mm == 0) atomicOr(const_cast<int *>(&prm->clamped), kClampParticle);
+    col = col < 0 ? 0 : m - 1;
+  }
+  UinT[idx] = U[(size_t)mm * prm->ldu + col];
 }
 
 // Laplace expansion, draw and output of step k for sample s (one wave): the k minors are
@@ -565,7 +587,7 @@ int32_t enqueue(const tb_gccb_config &c, char *ws, hipStream_t s) {
     tb::SmallPlan pl{};
     int ed_slots = 0;
     for (int k = 2; k <= kmax; ++k) {
-      tbs::sweep_plan_of(k, c.chunk_bits[k], &pl.c[k], &pl.wpi[k], &pl.iters[k]);
+      tbs::sweep_plan_of(k, S, c.chunk_bits[k], &pl.c[k], &pl.wpi[k], &pl.iters[k]);
       ed_slots = std::max(ed_slots, pl.c[k] - std::min(tb::kSweepU, k - 1));
     }
     const size_t small_lds = tb::gccb_small_lds(m, n, ed_slots);
@@ -615,7 +637,7 @@ std::map<int, hipStream_t> g_capture_streams;
 static tb_gccb_config resolve_chunks(const tb_gccb_config &cfg) {
   tb_gccb_config c = cfg;
   for (int k = 2; k <= std::min(c.n, TB_SWEEP_MAX_K); ++k)
-    if (c.chunk_bits[k] == 0) c.chunk_bits[k] = tbh::tuned(TB_TUNE_SWEEP, k);
+    if (c.chunk_bits[k] == 0) c.chunk_bits[k] = tbh::tuned(TB_TUNE_SWEEP, k, c.samples);
   return c;
 }
 
@@ -729,4 +751,24 @@ extern "C" int32_t tb_gccb_sample(const tb_gccb_config *cfg, const tb_complex128
     it = g_graphs.emplace(key, exec).first;
   }
   return tbh::check_hip(hipGraphLaunch(it->second, s), "hipGraphLaunch");
+}
+
+extern "C" int32_t tb_gccb_status(const void *workspace, void *stream) {
+  if (!workspace) return tbh::fail(TB_EINVAL, "gccb_status: workspace is NULL");
+  hipStream_t s = (hipStream_t)stream;
+  int clamped = 0;
+  const char *prm = (const char *)workspace;  // the parameter block opens the workspace
+  int32_t rc = tbh::check_hip(
+      hipMemcpyAsync(&clamped, prm + offsetof(tb::GccbParams, clamped), sizeof(int),
+                     hipMemcpyDeviceToHost, s),
+      "hipMemcpyAsync");
+  if (rc) return rc;
+  rc = tbh::check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (rc) return rc;
+  if (clamped)
+    return tbh::fail(TB_EINVAL, "gccb: %s%s out of range in the last call (clamped; its samples "
+                     "are not meaningful)",
+                     (clamped & tb::kClampParticle) ? "particles[] entries " : "",
+                     (clamped & tb::kClampOrder) ? "forced_order[] entries " : "");
+  return TB_OK;
 }

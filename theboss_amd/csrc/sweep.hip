@@ -77,16 +77,34 @@ struct SweepPlan {
 #ifndef TB_SWEEP_BUILTIN_MAX_C
 #define TB_SWEEP_BUILTIN_MAX_C 11
 #endif
+// Shortest built-in walk per lane between re-seeds (bits): below it the O(k^2) seed and the
+// per-chunk wave sums of the k minors outweigh the walk itself.
+#ifndef TB_SWEEP_BUILTIN_MIN_C
+#define TB_SWEEP_BUILTIN_MIN_C 5
+#endif
 constexpr int64_t kSweepMaxBlocks = 8192;  // 4 one-wave blocks per SIMD of 256 CUs x 4 SIMDs x 2
+// Waves a launch should provide before the built-in plan stops shortening the chunks: 4
+// per SIMD (one-wave blocks, 2 resident per SIMD, so two rounds and a short tail).
+constexpr int64_t kSweepWaveTarget = 4096;
 
-// Default chunk: 2^min(11, k-7) ranks per lane (>= 2^U), so an item has at least 64 chunks
-// (one wave) and a lane walks at most 2048 Gray steps between re-seeds. The tuner may pick
-// longer walks (fewer seedings and reductions) where the batch still fills the GPU.
-inline SweepPlan sweep_plan(int k, int c_override) {
+// Built-in chunk: the longest walk (at most 2^11 Gray steps per lane) that still gives the
+// launch kSweepWaveTarget waves, but not below 2^5 steps. A batch of 4096 k = 20 items
+// keeps 2^11 (4 waves per item); one k = 20 item gets 2^5 (256 waves) instead of a
+// 4-wave launch. The tuner may choose otherwise per (k, batch bucket).
+inline SweepPlan sweep_plan(int k, int64_t items, int c_override) {
   SweepPlan p{};
   const int nb = k - 1;                      // Gray bits (particle 0 is fixed)
   const int u = std::min(tb::kSweepU, nb);   // static bits of the kernel for this k
-  int c = c_override > 0 ? c_override : std::min(TB_SWEEP_BUILTIN_MAX_C, nb - 6);
+  int c;
+  if (c_override > 0) {
+    c = c_override;
+  } else {
+    int want_bits = 0;  // log2(waves per item that bring the launch to the target)
+    while (std::max<int64_t>(items, 1) << want_bits < kSweepWaveTarget && want_bits < 30)
+      ++want_bits;
+    c = std::min(TB_SWEEP_BUILTIN_MAX_C, nb - 6 - want_bits);
+    c = std::max(c, std::min(TB_SWEEP_BUILTIN_MIN_C, nb - 6));
+  }
   c = std::max(c, u);
   c = std::min(c, std::min(nb, tb::kSweepMaxC));
   p.c = c;
@@ -137,12 +155,12 @@ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t sweep_partial_bytes(int k, int items, int c_override) {
   if (k < 2 || items <= 0) return 0;
-  const SweepPlan p = sweep_plan(k, c_override);
+  const SweepPlan p = sweep_plan(k, items, c_override);
   return align256((size_t)items * p.wpi * k * sizeof(tb::cd));
 }
 
-void sweep_plan_of(int k, int c_override, int *c, int *wpi, int *iters) {
-  const SweepPlan p = sweep_plan(k, c_override);
+void sweep_plan_of(int k, int items, int c_override, int *c, int *wpi, int *iters) {
+  const SweepPlan p = sweep_plan(k, items, c_override);
   *c = p.c;
   *wpi = p.wpi;
   *iters = p.iters;
@@ -153,7 +171,7 @@ void sweep_plan_of(int k, int c_override, int *c, int *wpi, int *iters) {
 int32_t run_sweep_partial(int k, const tb::cd *Vt, int ldv, int64_t item_stride,
                           const int *item_map, const int *item_count, int items, int c_override,
                           tb::cd *partial, int *wpi, hipStream_t s) {
-  const SweepPlan p = sweep_plan(k, c_override);
+  const SweepPlan p = sweep_plan(k, items, c_override);
   *wpi = p.wpi;
   return dispatch_sweep(k, Vt, ldv, item_stride, item_map, item_count, items, p, partial, s);
 }
@@ -162,7 +180,7 @@ int32_t run_sweep_partial(int k, const tb::cd *Vt, int ldv, int64_t item_stride,
 int32_t run_sweep(int k, const tb::cd *Vt, int ldv, int64_t item_stride, const int *item_map,
                   const int *item_count, int items, int c_override, tb::cd *partial,
                   tb::cd *minors, hipStream_t s) {
-  const SweepPlan p = sweep_plan(k, c_override);
+  const SweepPlan p = sweep_plan(k, items, c_override);
   int32_t rc = dispatch_sweep(k, Vt, ldv, item_stride, item_map, item_count, items, p, partial, s);
   if (rc) return rc;
   const int total = items * k;
@@ -175,7 +193,7 @@ int32_t run_sweep(int k, const tb::cd *Vt, int ldv, int64_t item_stride, const i
 
 extern "C" size_t tb_sweep_workspace_size(int32_t k, int32_t batch) {
   if (k > TB_SWEEP_MAX_K) return 0;
-  return tbs::sweep_partial_bytes(k, batch, tbh::tuned(TB_TUNE_SWEEP, k));
+  return tbs::sweep_partial_bytes(k, batch, tbh::tuned(TB_TUNE_SWEEP, k, batch));
 }
 
 extern "C" int32_t tb_sweep_minors_batch(const tb_complex128 *Vt, int32_t k, int32_t ldv,
@@ -192,7 +210,7 @@ extern "C" int32_t tb_sweep_minors_batch(const tb_complex128 *Vt, int32_t k, int
   if (ldv < k - 1) return tbh::fail(TB_EINVAL, "sweep: ldv=%d < k-1=%d", ldv, k - 1);
   if (item_stride < (int64_t)k * ldv)
     return tbh::fail(TB_EINVAL, "sweep: item_stride=%lld < k*ldv", (long long)item_stride);
-  const int c = tbh::tuned(TB_TUNE_SWEEP, k);  // read once: size check and launch agree
+  const int c = tbh::tuned(TB_TUNE_SWEEP, k, batch);  // read once: size check and launch agree
   const size_t need = tbs::sweep_partial_bytes(k, batch, c);
   if (!workspace || workspace_bytes < need)
     return tbh::fail(TB_EWORKSPACE, "sweep: workspace %zu bytes < required %zu", workspace_bytes,

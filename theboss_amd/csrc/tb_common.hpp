@@ -124,6 +124,8 @@ __device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_read
 namespace tbh {
 int32_t fail(int32_t code, const char *fmt, ...);
 int32_t check_hip(hipError_t e, const char *what);
-// Tuning-table entry (0 = built-in plan) for TB_TUNE_* kernel `kernel` at `size`.
-int32_t tuned(int kernel, int size);
+// Tuning-table entry (0 = built-in plan) for TB_TUNE_* kernel `kernel` at `size` and
+// `batch` (the table keeps one entry per batch bucket, see tune_bucket).
+int32_t tuned(int kernel, int size, int64_t batch);
+int tune_bucket(int64_t batch);
 }  // namespace tbh
