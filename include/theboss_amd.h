@@ -112,6 +112,20 @@ int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t batch, int64_t
                        int32_t precision, tb_complex128 *out, void *workspace,
                        size_t workspace_bytes, void *stream);
 
+/* tb_glynn_batch without a workspace: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The scratch memory of the launches (per-wave partial
+ * sums, the FP32 copy of the matrices) is a static device array of
+ * TB_GLYNN_NOALLOC_SCRATCH_BYTES on each GPU the library runs on, held until the process
+ * ends; a call whose tb_glynn_workspace_size exceeds it returns TB_ERANGE (16 x n=32 needs
+ * 2 MiB). Same arguments, results and errors as tb_glynn_batch otherwise. The array is
+ * shared by every call on that GPU: calls that overlap in time on one GPU (other streams,
+ * other threads) race, so the caller orders them. */
+#define TB_GLYNN_NOALLOC_SCRATCH_BYTES (16u << 20)
+int32_t tb_glynn_batch_noalloc(const tb_complex128 *A, int32_t n, int32_t batch,
+                               int64_t matrix_stride, int32_t precision, tb_complex128 *out,
+                               void *stream);
+
 /* ---------------------------------------------------------------------------------
  * Multiplicity-aware permanents over Guan codes (reflected mixed-radix Gray codes).
  * Replaces RyserPermanentCalculator.compute_permanent and

@@ -65,6 +65,8 @@ def test_workspace_queries_are_host_only(lib):
     lambda l: l.tb_sweep_minors_batch(None, 41, 40, 41 * 40, 1, None, None, 0, None),
     lambda l: l.tb_perm_grad_batch(None, 50, 1, 2500, ctypes.c_void_p(8), None, None, 0, None),
     lambda l: l.tb_guan_perm_batch(None, 1, None, None, 1, None, None, 1, 1, 5, None, None),
+    # more scratch than the static array of the no-workspace variant holds
+    lambda l: l.tb_glynn_batch_noalloc(ctypes.c_void_p(8), 40, 256, 1600, 0, ctypes.c_void_p(8), None),
 ])
 def test_invalid_arguments_fail_before_launch(lib, call):
     from theboss_amd import _lib

@@ -30,6 +30,7 @@ TB_TUNE_MAX_SIZE = 48
 TB_TUNE_BATCH_BUCKETS = 16
 
 TB_GLYNN_MAX_N = 48
+TB_GLYNN_NOALLOC_SCRATCH_BYTES = 16 << 20
 TB_SWEEP_MAX_K = 40
 TB_GUAN_MAX_ROWS = 48
 TB_GUAN_MAX_COLS = 32
@@ -70,6 +71,7 @@ SIGNATURES = {
     "tb_tune_get": (_i32, [_i32, _i32, _i64]),
     "tb_glynn_workspace_size": (_sz, [_i32, _i32, _i32]),
     "tb_glynn_batch": (_i32, [_vp, _i32, _i32, _i64, _i32, _vp, _vp, _sz, _vp]),
+    "tb_glynn_batch_noalloc": (_i32, [_vp, _i32, _i32, _i64, _i32, _vp, _vp]),
     "tb_guan_perm_batch": (_i32, [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, _vp, _vp]),
     "tb_cc_workspace_size": (_sz, [_i32, _i32, _i32, _i32]),
     "tb_cc_submatrices_batch": (

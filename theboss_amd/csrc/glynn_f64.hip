@@ -1,5 +1,7 @@
 // FP64 Glynn: instantiates the launch templates for n = 2..TB_GLYNN_MAX_N and holds the
 // C entry points.
+#include <mutex>
+
 #include "glynn_impl.hpp"
 
 namespace tbgl {
@@ -85,4 +87,60 @@ extern "C" int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t bat
   if (rc) return rc;
   return tbgl::glynn_dispatch_f32(n, (const tb::cpx<float> *)A32, (int64_t)n * n, batch, o,
                                     partial, c_pref, s);
+}
+
+// ------------------------------------------------------------------------------------
+// tb_glynn_batch_noalloc: the same launches with their scratch memory (per-wave partial
+// sums, the FP32 copy of the matrices) in a static device array instead of a caller's
+// workspace, for callers that cannot allocate. `scratch` is the array's host-side name
+// only; its address on a device comes from hipGetSymbolAddress, taken once per device
+// with that device current, and that address is what the launches receive.
+// ------------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[TB_GLYNN_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_rc[kMaxDevices];
+std::once_flag g_scratch_once[kMaxDevices];
+
+// Address of `scratch` on the current device (resolved on the first call on it).
+int32_t scratch_on_current_device(void **ptr) {
+  int dev = 0;
+  int32_t rc = tbh::check_hip(hipGetDevice(&dev), "hipGetDevice");
+  if (rc) return rc;
+  if (dev < 0 || dev >= kMaxDevices)
+    return tbh::fail(TB_ERANGE, "glynn_noalloc: device %d beyond %d", dev, kMaxDevices - 1);
+  std::call_once(g_scratch_once[dev], [dev]() {
+    g_scratch_rc[dev] = hipGetSymbolAddress(&g_scratch_ptr[dev], HIP_SYMBOL(scratch));
+  });
+  rc = tbh::check_hip(g_scratch_rc[dev], "hipGetSymbolAddress(scratch)");
+  if (rc) return rc;
+  *ptr = g_scratch_ptr[dev];
+  return TB_OK;
+}
+}  // namespace
+
+extern "C" int32_t tb_glynn_batch_noalloc(const tb_complex128 *A, int32_t n, int32_t batch,
+                                          int64_t matrix_stride, int32_t precision,
+                                          tb_complex128 *out, void *stream) {
+  if (batch < 0) return tbh::fail(TB_EINVAL, "glynn_noalloc: batch=%d < 0", batch);
+  if (n > TB_GLYNN_MAX_N)
+    return tbh::fail(TB_ERANGE, "glynn_noalloc: n=%d exceeds TB_GLYNN_MAX_N=%d", n, TB_GLYNN_MAX_N);
+  const size_t need =
+      n < 0 ? 0 : glynn_workspace_bytes(n, batch, precision, tbh::tuned(TB_TUNE_GLYNN, n, batch));
+  if (need > (size_t)TB_GLYNN_NOALLOC_SCRATCH_BYTES)
+    return tbh::fail(TB_ERANGE,
+                     "glynn_noalloc: n=%d batch=%d needs %zu bytes of scratch, more than the "
+                     "static %u (split the batch or use tb_glynn_batch with a workspace)",
+                     n, batch, need, (unsigned)TB_GLYNN_NOALLOC_SCRATCH_BYTES);
+  void *ws = nullptr;
+  if (need > 0) {
+    const int32_t rc = scratch_on_current_device(&ws);
+    if (rc) return rc;
+  }
+  // The workspace entry point re-validates every argument and re-checks the size against
+  // the whole array, so a tuning entry changed in between cannot overrun it.
+  return tb_glynn_batch(A, n, batch, matrix_stride, precision, out, ws,
+                        ws ? (size_t)TB_GLYNN_NOALLOC_SCRATCH_BYTES : 0, stream);
 }
