@@ -50,7 +50,8 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the built-in chunk sizes instead of tuning the sweeps first")
     ap.add_argument("--workload", default="gccb",
-                    choices=["gccb", "glynn", "grad", "gcca", "lossy", "fit", "hist", "bobs", "exact"],
+                    choices=["gccb", "glynn", "grad", "gcca", "lossy", "fit", "hist", "bobs", "exact",
+                             "sweep_small", "glynn_fp32"],
                     help="gccb: the benchmark line (default); the others measure the "
                          "single-large-permanent, differentiation, GCC-A, uniform-loss, "
                          "interferometer-fit, outcome-histogram, BOBS and exact-distribution "

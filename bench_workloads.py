@@ -69,6 +69,63 @@ def clock_during(fn, seconds):
     return res, cycles / ticks * hz / 1e9
 
 
+class GuanFlops:
+    """Counts, inside the with-block, the multiplicity-aware algorithmic flops of every
+    tb_guan_perm_batch launch the calculators make: per item prod_d (e_d + 1) Guan codes,
+    each costing 2|T| + 6(sum t - 1) + 2 (|T| complex adds into the sums of the product
+    side's distinct modes, sum t - 1 complex multiplies, the accumulation), with e the
+    enumerated side's multiplicities and t the product side's. The count is summed on the
+    device and read once at the end."""
+
+    def __enter__(self):
+        T = D.torch()
+        self._orig = ops.guan_perm_batch
+        self._acc = T.zeros((), dtype=T.float64, device="cuda")
+        self.launches = 0
+        self.items = 0
+
+        def counted(U, out_modes, out_mult, in_modes, in_mult, formula=L.TB_FORMULA_RYSER):
+            codes = (in_mult.to(T.float64) + 1).prod(dim=1)
+            distinct = (out_mult > 0).sum(dim=1).to(T.float64)
+            parts = out_mult.sum(dim=1).to(T.float64)
+            self._acc += (codes * (2 * distinct + 6 * (parts - 1) + 2)).sum()
+            self.launches += 1
+            self.items += int(in_mult.shape[0])
+            return self._orig(U, out_modes, out_mult, in_modes, in_mult, formula)
+
+        ops.guan_perm_batch = counted
+        return self
+
+    def __exit__(self, *exc):
+        ops.guan_perm_batch = self._orig
+        self.flops = float(self._acc.item())
+        return False
+
+
+def sweep_small(checker):
+    """The C&C minors sweep as the drop-in submatrices calculator and the plain GCC-B path
+    launch it: one item, and small batches, at k = 20 (the batch-aware built-in plan cuts
+    the chunks until the launch has ~4096 waves, not below 2^5 Gray steps per lane)."""
+    T = D.torch()
+    lib = L.load()
+    k = 20
+    V = checker.haar_unitary(2 * k, 2020)[: k - 1, :k].T.copy()  # particle-major k x (k-1)
+    ref = ops.sweep_minors_batch(D.to_device_c128(V[None]), k).cpu().numpy()[0]
+    rows = []
+    for B in (1, 16, 256, 4096):
+        Vd = D.to_device_c128(np.ascontiguousarray(np.broadcast_to(V, (B, k, k - 1))))
+        reps = 200 if B <= 16 else 20
+        out = {}
+        t = _time(lambda: out.setdefault("m", ops.sweep_minors_batch(Vd, k)), reps=reps, warm_s=0.2)
+        wpi = lib.tb_sweep_workspace_size(k, B) // (B * k * 16)
+        fl = B * 2 ** (k - 1) * (10 * k - 12)
+        got = out["m"].cpu().numpy()
+        rows.append({"batch": B, "waves_per_item": int(wpi), "waves": int(B * wpi),
+                     "ms": t * 1e3, "tflops": fl / t / 1e12, "frac_fp64_peak": fl / t / FP64_PEAK,
+                     "max_rel_diff_vs_one_item": float(np.max(np.abs(got - ref[None])) / np.max(np.abs(ref)))})
+    return {"workload": "C&C minors sweep, k=20, batches 1..4096 (built-in plans)", "results": rows}
+
+
 def glynn_issue_ceiling(n, clock_ghz):
     """Fraction of the FP64 peak (2.4 GHz, every instruction an FMA) that the Glynn step can
     reach when its instructions issue back to back: 8n counted flops per step against
@@ -165,6 +222,30 @@ def glynn(checker, ns=(24, 28, 32), precisions=("fp64", "fp32_colsum"), cpu=True
                 if name in rec:
                     rec[name]["gpu_vs_cpu_baseline"] = wall_c / (rec[name]["ms"] * 1e-3)
     return {"workload": "single large permanents (Glynn, 16 per n)", "results": rows}
+
+
+def glynn_fp32(checker, sizes=((16, 4096), (20, 4096), (23, 512))):
+    """The FP32 column-sum variant where it is meant to be used (n <= 23, batches whose
+    launch lasts milliseconds), against the FP64 kernel on the same blocks: time, fraction
+    of the FP32 and FP64 vector peaks (8n 2^(n-1) flops per permanent), shader clock and
+    the max relative deviation from the FP64 result."""
+    rows = []
+    for n, B in sizes:
+        base = np.stack([checker.haar_unitary(2 * n, 7000 + n + b)[:n, :n] for b in range(16)])
+        A = D.to_device_c128(np.ascontiguousarray(np.resize(base, (B, n, n))))
+        fl = B * 8 * n * 2 ** (n - 1)
+        rec = {"n": n, "batch": B, "flops": fl}
+        for prec, name in ((L.TB_PREC_FP32_COLSUM, "fp32_colsum"), (L.TB_PREC_FP64, "fp64")):
+            out = {}
+            t = _time(lambda: out.setdefault("v", ops.glynn_batch(A, prec)), reps=10, warm_s=0.2)
+            _, ghz = clock_during(lambda: [ops.glynn_batch(A, prec) for _ in range(5)], 0.8 * 5 * t)
+            rec[name] = {"ms": t * 1e3, "tflops": fl / t / 1e12, "frac_fp32_peak": fl / t / FP32_PEAK,
+                         "frac_fp64_peak": fl / t / FP64_PEAK, "clock_ghz": ghz}
+            rec[name + "_values"] = out["v"].cpu().numpy()
+        v32, v64 = rec.pop("fp32_colsum_values"), rec.pop("fp64_values")
+        rec["fp32_max_rel_diff_vs_fp64"] = float(np.max(np.abs(v32 - v64) / np.abs(v64)))
+        rows.append(rec)
+    return {"workload": "Glynn FP32 column sums at n = 16, 20, 23 (FP64 alongside)", "results": rows}
 
 
 def grad_cpu(checker, mats):
@@ -284,16 +365,18 @@ def exact(checker, m=12, n=6, eta=0.5):
     T = D.torch()
     run()  # warm (library, workspaces)
     T.cuda.synchronize()
-    t0 = time.perf_counter()
-    dist = run()
-    T.cuda.synchronize()
-    dt = time.perf_counter() - t0
+    with GuanFlops() as gf:
+        t0 = time.perf_counter()
+        dist = run()
+        T.cuda.synchronize()
+        dt = time.perf_counter() - t0
     perms = sum(math.comb(l + m - 1, l) * math.comb(n, l) for l in range(1, n + 1))
     t0 = time.perf_counter()
     ref = np.asarray(checker.uniform_losses_distribution(U, inp, eta, checker.perm_ryser))
     dt_cpu = time.perf_counter() - t0
     return {"workload": f"exact uniform-loss distribution, m={m}, n={n}, eta={eta}",
             "outcomes": len(dist), "permanents": perms, "seconds": dt,
+            "guan_launches": gf.launches, "guan_items": gf.items, "guan_flops": gf.flops,
             "permanents_per_s": perms / dt, "sum": float(dist.sum()),
             "max_abs_diff_vs_oracle": float(np.max(np.abs(dist - ref))),
             "cpu_baseline": {"value": perms / dt_cpu, "unit": "permanents/s", "cores": 1,
@@ -322,10 +405,11 @@ def gcca(checker):
     strat = GeneralizedCliffordsSimulationStrategy(GPURyserPermanentCalculator(U),
                                                    rng=np.random.default_rng(0))
     D.torch().cuda.synchronize()
-    t0 = time.perf_counter()
-    samples = strat.simulate(inp, S)
-    D.torch().cuda.synchronize()
-    dt = time.perf_counter() - t0
+    with GuanFlops() as gf:
+        t0 = time.perf_counter()
+        samples = strat.simulate(inp, S)
+        D.torch().cuda.synchronize()
+        dt = time.perf_counter() - t0
     ones = [1] * n + [0] * (m - n)
     r = GPURyserPermanentCalculator(U, ones, ones).compute_permanent()
     g = GPUGlynnGrayPermanentCalculator(U, ones, ones).compute_permanent()
@@ -352,6 +436,7 @@ def gcca(checker):
                      f"{spent:.1f} s, scaled by this run's {len(strat.pmfs)} layers"}
     return {"workload": "GCC-A m=20 n=10, 200 samples (Ryser, lock-step layers)",
             "samples_per_s": S / dt, "seconds": dt, "layers_computed": len(strat.pmfs),
+            "guan_launches": gf.launches, "guan_items": gf.items, "guan_flops": gf.flops,
             "cpu_python_baseline": cpu, "gpu_vs_cpu_python": (S / dt) / cpu["value"],
             "ryser_vs_glynn_10x10_rel": abs(r - g) / abs(g),
             "all_samples_have_n_photons": all(sum(s_) == n for s_ in samples)}
