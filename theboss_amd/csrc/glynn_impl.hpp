@@ -233,11 +233,13 @@ __global__ __launch_bounds__(256) TB_WIDE_ATTR(N) void glynn_wide_kernel(const c
   __syncthreads();
   double are = 0.0, aim = 0.0;
   // R0 = 0: row 0's first values in SGPRs; 1: row 0 in VGPRs; 2 (FP64): rows 0..U+1
-  // (n <= TB_GLYNN_BCAST_WIDE_MAX_N) or 0..U-1 in broadcast form.
+  // (n <= TB_GLYNN_BCAST_WIDE_MAX_N) or 0..U-1 in broadcast form; 3: rows 0..U-1 (every
+  // static flip) in VGPRs.
   constexpr int S = (R0 == 0) ? TB_GLYNN_SROW : 0;
   constexpr int BW = (N <= TB_GLYNN_BCAST_WIDE_MAX_N) ? U + 2 : U;  // rows in broadcast form
   constexpr int B = (R0 == 2) ? (BW < N - 1 ? BW : N - 1) : 0;
-  glynn_lane<N, U, (R0 == 2 ? 0 : R0), F, const cpx<F> *, (S < N ? S : N), B>(
+  constexpr int RR = (R0 == 2) ? 0 : (R0 == 3 ? U : R0);  // rows in plain VGPRs
+  glynn_lane<N, U, RR, F, const cpx<F> *, (S < N ? S : N), B>(
       (const cpx<F> *)Ms, C, (uint64_t)wq * 64 + lane, (uint64_t)wpm * 64, iters, are, aim);
   are = wave_sum(are);
   aim = wave_sum(aim);
@@ -316,10 +318,16 @@ namespace tbgl {
 #ifndef TB_GLYNN_F64_BCAST_MAX_N
 #define TB_GLYNN_F64_BCAST_MAX_N 40
 #endif
+// FP32 rows cost half the registers: up to TB_GLYNN_F32_ROWS_MAX_N the FP32 kernels keep
+// every static row (0..U-1) in VGPRs, so only block-start flips read LDS.
+#ifndef TB_GLYNN_F32_ROWS_MAX_N
+#define TB_GLYNN_F32_ROWS_MAX_N 24
+#endif
 template <int N, typename F>
 constexpr int glynn_reg_rows() {
   if (sizeof(F) == 8 && N <= TB_GLYNN_F64_BCAST_MAX_N) return 2;
-  return (sizeof(F) == 4 || N <= TB_GLYNN_F64_R0_MAX_N) ? 1 : 0;
+  if (sizeof(F) == 4) return N <= TB_GLYNN_F32_ROWS_MAX_N ? 3 : 1;
+  return N <= TB_GLYNN_F64_R0_MAX_N ? 1 : 0;
 }
 
 // Static Gray bits (unrolled block of 2^U steps) of the WIDE kernels: FP64 at n >= 25,
@@ -327,11 +335,23 @@ constexpr int glynn_reg_rows() {
 #ifndef TB_GLYNN_U_LARGE
 #define TB_GLYNN_U_LARGE 3
 #endif
+// Static Gray bits of the FP32 kernels that keep their static rows in VGPRs: 3 (7/8 of
+// the flips from registers; measured 1.05x at n = 20, 1.11x at n = 23 over row 0 alone).
+// The plan keeps at least 2^8 chunks per matrix (C <= n - 9) and needs C >= U, so U = 3
+// starts at n = 12.
+#ifndef TB_GLYNN_F32_U
+#define TB_GLYNN_F32_U 3
+#endif
 template <int N, typename F>
 constexpr int glynn_u() {
-  return (sizeof(F) == 8 && N >= 25) ? TB_GLYNN_U_LARGE : 2;
+  if (sizeof(F) == 4)
+    return (N <= TB_GLYNN_F32_ROWS_MAX_N && N - 9 >= TB_GLYNN_F32_U) ? TB_GLYNN_F32_U : 2;
+  return N >= 25 ? TB_GLYNN_U_LARGE : 2;
 }
-inline int glynn_u_max(int n) { return n >= 25 ? std::max(2, TB_GLYNN_U_LARGE) : 2; }
+inline int glynn_u_max(int n) {
+  const int u32 = (n <= TB_GLYNN_F32_ROWS_MAX_N && n - 9 >= TB_GLYNN_F32_U) ? TB_GLYNN_F32_U : 2;
+  return std::max({2, n >= 25 ? TB_GLYNN_U_LARGE : 2, u32});
+}
 
 constexpr int kWideMinN = 11;  // below this a lane owns a whole matrix
 constexpr int kMaxC = 12;      // Gray bits per lane chunk
@@ -381,6 +401,7 @@ inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, int batch, t
     return tbh::check_hip(hipGetLastError(), "glynn_narrow_kernel");
   } else {
     constexpr int U = glynn_u<N, F>();
+    static_assert(U <= N - 9, "the static block must fit the smallest chunk (C <= n - 9)");
     const GlynnPlan p = glynn_plan(N, batch, c_pref, U);
     const unsigned blocks = (unsigned)batch * (unsigned)(p.wpm / 4);
     hipLaunchKernelGGL((tb::glynn_wide_kernel<N, U, glynn_reg_rows<N, F>(), F>), dim3(blocks), dim3(256), 0, s,
