@@ -56,9 +56,11 @@ def main() -> int:
                          "single-large-permanent, differentiation, GCC-A, uniform-loss, "
                          "interferometer-fit, outcome-histogram, BOBS and exact-distribution "
                          "workloads and print their own JSON line")
-    ap.add_argument("--pmc-probe", action="store_true",
-                    help="only run the k=20 sweep twice and a 256 MiB device copy (the "
-                         "calibration reference) for rocprofv3 --pmc passes")
+    ap.add_argument("--pmc-probe", nargs="?", const="sweep", choices=["sweep", "grad"],
+                    help="only run kernels for rocprofv3 --pmc passes: 'sweep' (default) the k=20 "
+                         "sweep twice and a 256 MiB device copy (the calibration reference); 'grad' "
+                         "the n=20 gradient of the 64 differentiation blocks three times (chunk "
+                         "2^9) and one k=20 sweep over 4096 items")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -83,6 +85,21 @@ def main() -> int:
         return 0
     U = orc.haar_unitary(M, SEED)
     Ud = D.to_device_c128(U, dev)
+    if args.pmc_probe == "grad":
+        n, B = 20, 64
+        mats = []
+        for b in range(B):
+            rows = np.sort(np.random.default_rng(300 + b).choice(40, n, replace=False))
+            mats.append(orc.haar_unitary(40, 300 + b)[rows, :n])
+        A = D.to_device_c128(np.stack(mats), dev)
+        autotune.set_entry("grad", n, 9, B)
+        for _ in range(3):
+            ops.perm_grad_batch(A)
+        Vt = Ud[:N, :N].T.contiguous().unsqueeze(0).expand(SAMPLES, N, N).contiguous()
+        ops.sweep_minors_batch(Vt, N)
+        torch.cuda.synchronize()
+        print("pmc probe done")
+        return 0
     if args.pmc_probe:
         Vt = Ud[:N, :N].T.contiguous().unsqueeze(0).expand(SAMPLES, N, N).contiguous()
         for _ in range(2):
