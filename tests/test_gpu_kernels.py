@@ -165,6 +165,82 @@ def test_guan_outer_batch_vs_oracle(T, oracle):
         GPURyserPermanentCalculator(U).compute_permanent_outer(S, np.array([[1, 0, 0, 0, 0, 0]]))
 
 
+def _occupations(rng, m, ks):
+    return np.array([np.bincount(rng.integers(0, m, int(k)), minlength=m) for k in ks])
+
+
+@pytest.mark.parametrize("formula", [0, 1])
+def test_guan_packed_waves_vs_one_item_per_wave(T, ops, oracle, formula):
+    """A launch of >= 16384 items runs four items per wave: 16 lanes per item, or all 64
+    lanes in turn for the wave's items when one of them has more than 1024 Guan codes;
+    smaller launches run one item per wave. The same items in one launch of 16387 (not a
+    multiple of 4) and in launches of 4000 agree to rounding (the rank ranges differ), and
+    a strided subset (to 1e-12) plus every large item (to the relative FP64 bar) agree
+    with the oracle. Two batches: 1..7
+    particles (at most 7 rows, the 8-row kernel), then the same with 40 scattered
+    16-particle items of 2048 codes (the 16-row kernel, both wave paths)."""
+    from theboss_amd.states import compact_batch
+
+    rng = np.random.default_rng(17 + formula)
+    m, B = 12, 16387
+    U = oracle.haar_unitary(m, 77)
+    Ud = dev_c(T, U)
+    ks = rng.integers(1, 8, B)
+    S, Tm = _occupations(rng, m, ks), _occupations(rng, m, ks)
+    for with_big in (False, True):
+        S2, T2 = S.copy(), Tm.copy()
+        big = np.sort(rng.choice(B, 40, replace=False)) if with_big else np.zeros(0, int)
+        for b in big:
+            S2[b] = 0
+            S2[b, :6] = [3, 3, 3, 3, 3, 1]
+            T2[b] = np.bincount(rng.integers(0, m, 16), minlength=m)
+        im, iu = compact_batch(S2)
+        om, ou = compact_batch(T2)
+        assert (om.shape[1] <= 8) != with_big
+        args = [dev_i(T, a) for a in (om, ou, im, iu)]
+        full = ops.guan_perm_batch(Ud, *args, formula=formula).cpu().numpy()
+        parts = np.concatenate([ops.guan_perm_batch(Ud, *[a[i:i + 4000] for a in args],
+                                                    formula=formula).cpu().numpy()
+                                for i in range(0, B, 4000)])
+        assert np.max(np.abs(full - parts)) <= 1e-12, with_big
+        small = np.setdiff1d(np.arange(0, B, 97), big)
+        ref = np.array([oracle.perm_classic(U, S2[b], T2[b]) for b in small])
+        assert np.max(np.abs(full[small] - ref)) <= 1e-12, with_big
+        if with_big:  # 2048-term sums with binomial weights up to 3^5: the FP64 bar
+            ref = np.array([oracle.perm_glynn(U, S2[b], T2[b]) for b in big])
+            assert np.max(np.abs(full[big] - ref) / np.abs(ref)) <= REL
+
+
+def test_cc_minors_collision_inputs_packed_waves(T, ops, oracle):
+    """The C&C minors of collision inputs (mixed-radix kernel, minors mode) in one launch
+    of 16387 items of 12 particles (codes from 2^5 to 2^12: packed waves and waves that
+    walk their items in turn) against launches of 4000 and, on a strided subset, the
+    oracle, to the relative FP64 bar; slots of an item beyond its occupied modes hold 0."""
+    from theboss_amd.states import compact_batch
+
+    rng = np.random.default_rng(23)
+    m, B, k = 12, 16387, 12
+    U = oracle.haar_unitary(m, 78)
+    S = _occupations(rng, m, np.full(B, k))
+    Tm = _occupations(rng, m, np.full(B, k - 1))
+    codes = np.prod(S + 1, axis=1)
+    assert codes.min() <= 1024 < codes.max()
+    im, iu = compact_batch(S)
+    om, ou = compact_batch(Tm)
+    args = [dev_i(T, a) for a in (om, ou, im, iu)]
+    Ud = dev_c(T, U)
+    full = ops.cc_submatrices_batch(Ud, *args, k, False).cpu().numpy()
+    parts = np.concatenate([ops.cc_submatrices_batch(Ud, *[a[i:i + 4000] for a in args], k,
+                                                     False).cpu().numpy()
+                            for i in range(0, B, 4000)])
+    assert rel(full, parts) <= REL  # 16 against 64 lanes per item: another summation order
+    assert np.all(full[iu == 0] == 0)
+    for b in range(0, B, 211):
+        ref = oracle.cc_submatrices(U, S[b], Tm[b])
+        live = iu[b] > 0
+        assert rel(full[b][live], ref[im[b][live]]) <= REL, b
+
+
 def test_shape_mismatch_raises_attribute_error(T, oracle):
     from theboss_amd.permanent_calculators import GPURyserPermanentCalculator
 

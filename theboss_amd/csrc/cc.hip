@@ -20,7 +20,6 @@ namespace tbg {
 int32_t run_guan(const tb::cd *U, int ldu, const int *out_modes, const int *out_mult, int out_max,
                  const int *in_modes, const int *in_mult, int in_max, int batch, int formula,
                  int mode, tb::cd *out, hipStream_t s);
-size_t guan_lds_bytes(int out_max, int in_max, int mode);
 }  // namespace tbg
 
 namespace tb {

@@ -11,13 +11,20 @@
 //    bs_cc_ryser_submatrices_permanent_calculator.py:80-119):
 //      minor_d = sum_{r<=s, r != 0, r_d < s_d} (-1)^(|t|+|r|) prod C(s,r) (s_d-r_d)/s_d prod_j (...)^t_j
 //
-// Work split: one wave per item; each lane walks a contiguous range of Guan ranks. A
-// lane seeds its Guan state at its first rank from the mixed-radix digits (digit d is
-// reflected when the number formed by the higher digits is odd, and its direction is
-// the direction of that sweep), then follows the reference's update rule
-// (bs_permanent_calculator_base.py:123-149), so every lane visits exactly the codes the
-// reference visits at those ranks. The binomial product is kept exact: it is an integer
-// below 2^53, updated by exact division and multiplication with table binomials.
+// Work split: an item's Guan ranks are divided into contiguous ranges, one per lane of
+// the W lanes that own the item. A lane seeds its Guan state at its first rank from the
+// mixed-radix digits (digit d is reflected when the number formed by the higher digits is
+// odd, and its direction is the direction of that sweep), then follows the reference's
+// update rule (bs_permanent_calculator_base.py:123-149), so every lane visits exactly the
+// codes the reference visits at those ranks. The binomial product is kept exact: it is an
+// integer below 2^53, updated by exact division and multiplication with table binomials.
+//
+// Launch shapes. Large batches of small items (GCC-A layers: about 100 codes per item)
+// are latency-bound, so a wave takes IPW = 4 consecutive items: the lanes read the
+// items' occupation slots in parallel and compact them with ballots, and when every item
+// of the wave has at most kGuanPackedMaxCodes codes each item is walked by its own 16
+// lanes; otherwise the wave walks its items one after another with all 64 lanes. Small
+// batches keep one item per wave (IPW = 1, W = 64).
 #include <algorithm>
 
 #include "tb_common.hpp"
@@ -33,112 +40,98 @@ __device__ __forceinline__ double binom_exact(int n, int k) {
   return rint(r);
 }
 
-struct GuanItem {
-  int D, J, sum_s, sum_t;
+// A packed wave walks each of its items with 16 lanes when every item has at most this
+// many Guan codes (at most 64 codes per lane).
+constexpr int kGuanPackedMaxCodes = 1024;
+
+// Per-item LDS block: X[J][D] (compact rows x compact columns) for at most out_max x
+// in_max, then the compact input multiplicities s, their slots and modes, and the compact
+// output multiplicities t and modes.
+struct GuanItemLds {
+  cd *X;
+  int *s, *slot, *cmode, *t, *rmode;
 };
 
-// mode 0: single permanent (formula 0 Ryser / 1 Chin-Huh) -> out[b]
-// mode 1: C&C minors (Ryser) -> out[b*in_max + slot]
-template <int JB>
-__global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int ldu,
-                                                  const int *__restrict__ out_modes,
-                                                  const int *__restrict__ out_mult, int out_max,
-                                                  const int *__restrict__ in_modes,
-                                                  const int *__restrict__ in_mult, int in_max,
-                                                  int formula, int mode, cd *__restrict__ out) {
-  extern __shared__ char smem[];
-  const int b = blockIdx.x;
-  const int lane = threadIdx.x;
-  // LDS layout: X[J][D] cd | minors acc [D][64] cd | s[D] t[J] dslot[D] int | r,u [D][64] int8
-  cd *X = (cd *)smem;
-  cd *Macc = X + out_max * in_max;
-  int *s_d = (int *)(Macc + (mode == 1 ? in_max * 64 : 0));
-  int *t_j = s_d + in_max;
-  int *dslot = t_j + out_max;
-  int *hdr = dslot + in_max;  // D, J, sum_s, sum_t
-  signed char *rr = (signed char *)(hdr + 4);
-  signed char *uu = rr + in_max * 64;
+__host__ __device__ __forceinline__ size_t guan_item_stride(int out_max, int in_max) {
+  const size_t bytes = sizeof(cd) * (size_t)out_max * in_max +
+                       sizeof(int) * (size_t)(3 * in_max + 2 * out_max);
+  return (bytes + 15) & ~(size_t)15;
+}
 
-  if (lane == 0) {
-    int D = 0, J = 0, ss = 0, st = 0;
-    for (int d = 0; d < in_max; ++d) {
-      const int m = in_mult[(size_t)b * in_max + d];
-      if (m > 0) {
-        s_d[D] = m;
-        dslot[D] = d;
-        ++D;
-        ss += m;
-      }
+__device__ __forceinline__ GuanItemLds guan_item_at(char *p, int out_max, int in_max) {
+  GuanItemLds it;
+  it.X = (cd *)p;
+  it.s = (int *)(it.X + (size_t)out_max * in_max);
+  it.slot = it.s + in_max;
+  it.cmode = it.slot + in_max;
+  it.t = it.cmode + in_max;
+  it.rmode = it.t + out_max;
+  return it;
+}
+
+// Sums over aligned groups of W lanes; every lane of a group receives its group's sum.
+template <int W>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = W / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+template <int W>
+__device__ __forceinline__ int group_sum_i(int v) {
+#pragma unroll
+  for (int off = W / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Guan state at rank lo: digits rr[d*64 + lane], directions uu[d*64 + lane], their sum and
+// the binomial product. One division per digit; R = uint32_t when the item has fewer
+// than 2^32 codes.
+template <typename R>
+__device__ __forceinline__ void guan_seed(R lo, int D, const int *s, signed char *rr,
+                                          signed char *uu, int lane, int &sum_r, double &binom) {
+  R q = lo;
+  for (int d = 0; d < D; ++d) {
+    const R bd = (R)(s[d] + 1);
+    const R high = q / bd;
+    const int a = (int)(q - high * bd);
+    int g, u;
+    if ((high & 1) == 0) {
+      g = a;
+      u = 1;
+    } else {
+      g = (int)bd - 1 - a;
+      u = -1;
     }
-    for (int j = 0; j < out_max; ++j) {
-      const int m = out_mult[(size_t)b * out_max + j];
-      if (m > 0) {
-        t_j[J] = m;
-        ++J;
-        st += m;
-      }
-    }
-    hdr[0] = D;
-    hdr[1] = J;
-    hdr[2] = ss;
-    hdr[3] = st;
+    rr[d * 64 + lane] = (signed char)g;
+    uu[d * 64 + lane] = (signed char)u;
+    sum_r += g;
+    binom *= binom_exact(s[d], g);
+    q = high;
   }
-  __syncthreads();
-  const int D = hdr[0], J = hdr[1], sum_s = hdr[2], sum_t = hdr[3];
-  // Gather X[j][d] = U[out_mode_j][in_mode_d] over the compact (nonzero) slots.
-  for (int e = lane; e < J * D; e += 64) {
-    const int j = e / D, d = e - j * D;
-    int jo = -1, cnt = 0;
-    for (int q = 0; q < out_max; ++q)
-      if (out_mult[(size_t)b * out_max + q] > 0) {
-        if (cnt == j) {
-          jo = q;
-          break;
-        }
-        ++cnt;
-      }
-    cd v = {0.0, 0.0};
-    if (jo >= 0) {
-      const int rowm = out_modes[(size_t)b * out_max + jo];
-      const int colm = in_modes[(size_t)b * in_max + dslot[d]];
-      v = U[(size_t)rowm * ldu + colm];
-    }
-    X[e] = v;
-  }
+}
+
+// Walks, reduces and stores one item with the W lanes of group l's item (l = lane index
+// inside the group). mode 0: single permanent (formula 0 Ryser / 1 Chin-Huh) -> out[b];
+// mode 1: C&C minors (Ryser) -> out[b*in_max + slot] for the item's nonzero slots.
+template <int JB, int W>
+__device__ __forceinline__ void guan_item_run(const GuanItemLds &it, int D, int J, int sum_s,
+                                              int sum_t, uint64_t total, int b, int in_max,
+                                              int l, int lane, int formula, int mode, cd *Macc,
+                                              signed char *rr, signed char *uu,
+                                              cd *__restrict__ out) {
   if (mode == 1)
-    for (int e = lane; e < D * 64; e += 64) Macc[e] = {0.0, 0.0};
-  __syncthreads();
-
-  // Total ranks and this lane's range.
-  uint64_t total = 1;
-  for (int d = 0; d < D; ++d) total *= (uint64_t)(s_d[d] + 1);
-  const uint64_t per = (total + 63) / 64;
-  const uint64_t lo = (uint64_t)lane * per;
+    for (int d = 0; d < D; ++d) Macc[d * 64 + lane] = {0.0, 0.0};
+  const uint64_t per = (total + W - 1) / W;
+  const uint64_t lo = (uint64_t)l * per;
   const uint64_t hi = min(total, lo + per);
   double acc_re = 0.0, acc_im = 0.0;
   if (lo < hi) {
-    // Seed digits/directions at rank lo.
-    uint64_t P = 1;
     int sum_r = 0;
     double binom = 1.0;
-    for (int d = 0; d < D; ++d) {
-      const uint64_t bd = (uint64_t)(s_d[d] + 1);
-      const int a = (int)((lo / P) % bd);
-      const uint64_t high = lo / (P * bd);
-      int g, u;
-      if ((high & 1ull) == 0) {
-        g = a;
-        u = 1;
-      } else {
-        g = (int)bd - 1 - a;
-        u = -1;
-      }
-      rr[d * 64 + lane] = (signed char)g;
-      uu[d * 64 + lane] = (signed char)u;
-      sum_r += g;
-      binom *= binom_exact(s_d[d], g);
-      P *= bd;
-    }
+    if (total <= 0xffffffffull)
+      guan_seed<uint32_t>((uint32_t)lo, D, it.s, rr, uu, lane, sum_r, binom);
+    else
+      guan_seed<uint64_t>(lo, D, it.s, rr, uu, lane, sum_r, binom);
     double sr[JB], si[JB];
 #pragma unroll
     for (int j = 0; j < JB; ++j) {
@@ -147,18 +140,18 @@ __global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int 
     }
     for (int d = 0; d < D; ++d) {
       const int g = rr[d * 64 + lane];
-      const double coef = (formula == 0) ? (double)g : (double)(s_d[d] - 2 * g);
+      const double coef = (formula == 0) ? (double)g : (double)(it.s[d] - 2 * g);
 #pragma unroll
       for (int j = 0; j < JB; ++j)
         if (j < J) {
-          const cd x = X[j * D + d];
+          const cd x = it.X[j * D + d];
           sr[j] = fma(coef, x.re, sr[j]);
           si[j] = fma(coef, x.im, si[j]);
         }
     }
     // Sign: Ryser perm (-1)^(|s|+|r|); Chin-Huh (-1)^|r| (scale 2^-|s| at the end);
     // C&C minors (-1)^(|t|+|r|).
-    int base_par = (mode == 1) ? sum_t : (formula == 0 ? sum_s : 0);
+    const int base_par = (mode == 1) ? sum_t : (formula == 0 ? sum_s : 0);
     double sgn = ((base_par + sum_r) & 1) ? -1.0 : 1.0;
     for (uint64_t rank = lo;;) {
       // Term at the current code.
@@ -166,7 +159,7 @@ __global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int 
 #pragma unroll
       for (int j = 0; j < JB; ++j)
         if (j < J)
-          for (int e = 0; e < t_j[j]; ++e) p = cmul(p, cd{sr[j], si[j]});
+          for (int e = 0; e < it.t[j]; ++e) p = cmul(p, cd{sr[j], si[j]});
       const double w = sgn * binom;
       if (mode == 0) {
         acc_re = fma(w, p.re, acc_re);
@@ -174,8 +167,8 @@ __global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int 
       } else if (rank != 0) {
         for (int d = 0; d < D; ++d) {
           const int g = rr[d * 64 + lane];
-          if (g < s_d[d]) {
-            const double f = w * (double)(s_d[d] - g) / (double)s_d[d];
+          if (g < it.s[d]) {
+            const double f = w * (double)(it.s[d] - g) / (double)it.s[d];
             cd m = Macc[d * 64 + lane];
             m.re = fma(f, p.re, m.re);
             m.im = fma(f, p.im, m.im);
@@ -189,7 +182,7 @@ __global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int 
       int k = rr[lane] + uu[lane];
       // A movable digit exists because rank < total; the d + 1 < D bound only keeps the
       // scan inside the lane's state.
-      while ((k > s_d[d] || k < 0) && d + 1 < D) {
+      while ((k > it.s[d] || k < 0) && d + 1 < D) {
         uu[d * 64 + lane] = (signed char)(-uu[d * 64 + lane]);
         ++d;
         k = rr[d * 64 + lane] + uu[d * 64 + lane];
@@ -197,12 +190,12 @@ __global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int 
       const int old = rr[d * 64 + lane];
       rr[d * 64 + lane] = (signed char)k;
       const int dr = k - old;
-      binom = binom / binom_exact(s_d[d], old) * binom_exact(s_d[d], k);
+      binom = binom / binom_exact(it.s[d], old) * binom_exact(it.s[d], k);
       const double coef = (formula == 0) ? (double)dr : -2.0 * (double)dr;
 #pragma unroll
       for (int j = 0; j < JB; ++j)
         if (j < J) {
-          const cd x = X[j * D + d];
+          const cd x = it.X[j * D + d];
           sr[j] = fma(coef, x.re, sr[j]);
           si[j] = fma(coef, x.im, si[j]);
         }
@@ -210,22 +203,113 @@ __global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int 
     }
   }
   if (mode == 0) {
-    acc_re = wave_sum(acc_re);
-    acc_im = wave_sum(acc_im);
-    if (lane == 0) {
+    acc_re = group_sum<W>(acc_re);
+    acc_im = group_sum<W>(acc_im);
+    if (l == 0) {
       const double scale = (formula == 1) ? ldexp(1.0, -sum_s) : 1.0;
       out[b] = {scale * acc_re, scale * acc_im};
     }
   } else {
-    __syncthreads();
-    for (int q = 0; q < in_max; ++q)
-      if (lane == 0) out[(size_t)b * in_max + q] = {0.0, 0.0};
-    __syncthreads();
     for (int d = 0; d < D; ++d) {
-      const double re = wave_sum(Macc[d * 64 + lane].re);
-      const double im = wave_sum(Macc[d * 64 + lane].im);
-      if (lane == 0) out[(size_t)b * in_max + dslot[d]] = {re, im};
+      const double re = group_sum<W>(Macc[d * 64 + lane].re);
+      const double im = group_sum<W>(Macc[d * 64 + lane].im);
+      if (l == 0) out[(size_t)b * in_max + it.slot[d]] = {re, im};
     }
+  }
+}
+
+// LDS: Macc[in_max][64] cd (mode 1) | IPW item blocks | rr, uu [in_max][64] int8.
+template <int JB, int IPW>
+__global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int ldu,
+                                                  const int *__restrict__ out_modes,
+                                                  const int *__restrict__ out_mult, int out_max,
+                                                  const int *__restrict__ in_modes,
+                                                  const int *__restrict__ in_mult, int in_max,
+                                                  int batch, int formula, int mode,
+                                                  cd *__restrict__ out) {
+  constexpr int W = 64 / IPW;  // lanes per item in a packed wave
+  extern __shared__ char smem[];
+  const int lane = threadIdx.x;
+  const int g = lane / W, l = lane % W;
+  const int b0 = blockIdx.x * IPW;
+  const int b = b0 + g;
+  const bool live = b < batch;
+  cd *Macc = (cd *)smem;
+  char *items = smem + (mode == 1 ? sizeof(cd) * (size_t)in_max * 64 : 0);
+  const size_t stride = guan_item_stride(out_max, in_max);
+  signed char *rr = (signed char *)(items + IPW * stride);
+  signed char *uu = rr + in_max * 64;
+  const GuanItemLds it = guan_item_at(items + g * stride, out_max, in_max);
+
+  // Compact the nonzero slots of both sides in slot order: the group's lanes read W slots
+  // at a time and place each nonzero one by a ballot prefix count. In mode 1 a zero input
+  // slot's minor is 0 and is stored here; the nonzero slots are stored after the walk.
+  const uint64_t gmask = (W == 64) ? ~0ull : (((1ull << W) - 1ull) << (g * W));
+  const uint64_t below = (1ull << lane) - 1ull;
+  int D = 0, J = 0, ss = 0, st = 0;
+  for (int q0 = 0; q0 < in_max; q0 += W) {
+    const int q = q0 + l;
+    int m = 0, md = 0;
+    if (live && q < in_max) {
+      m = in_mult[(size_t)b * in_max + q];
+      md = in_modes[(size_t)b * in_max + q];
+      if (mode == 1 && m <= 0) out[(size_t)b * in_max + q] = {0.0, 0.0};
+    }
+    const uint64_t bal = __ballot(m > 0) & gmask;
+    if (m > 0) {
+      const int p = D + __popcll(bal & below);
+      it.s[p] = m;
+      it.slot[p] = q;
+      it.cmode[p] = md;
+      ss += m;
+    }
+    D += __popcll(bal);
+  }
+  for (int q0 = 0; q0 < out_max; q0 += W) {
+    const int q = q0 + l;
+    int m = 0, md = 0;
+    if (live && q < out_max) {
+      m = out_mult[(size_t)b * out_max + q];
+      md = out_modes[(size_t)b * out_max + q];
+    }
+    const uint64_t bal = __ballot(m > 0) & gmask;
+    if (m > 0) {
+      const int p = J + __popcll(bal & below);
+      it.t[p] = m;
+      it.rmode[p] = md;
+      st += m;
+    }
+    J += __popcll(bal);
+  }
+  ss = group_sum_i<W>(ss);
+  st = group_sum_i<W>(st);
+  __syncthreads();
+  // Gather X[j][d] = U[row mode j][column mode d] over the compact slots.
+  for (int e = l; e < J * D; e += W) {
+    const int j = e / D, d = e - j * D;
+    it.X[e] = U[(size_t)it.rmode[j] * ldu + it.cmode[d]];
+  }
+  __syncthreads();
+  uint64_t total = 1;
+  for (int d = 0; d < D; ++d) total *= (uint64_t)(it.s[d] + 1);
+
+  if (IPW == 1 || __all(!live || total <= (uint64_t)kGuanPackedMaxCodes)) {
+    if (live)
+      guan_item_run<JB, W>(it, D, J, ss, st, total, b, in_max, l, lane, formula, mode, Macc, rr,
+                           uu, out);
+    return;
+  }
+  // A large item among the wave's items: walk them one after another with all lanes.
+  for (int q = 0; q < IPW; ++q) {
+    const int bq = b0 + q;
+    if (bq >= batch) break;
+    const int Dq = __shfl(D, q * W, 64), Jq = __shfl(J, q * W, 64);
+    const int ssq = __shfl(ss, q * W, 64), stq = __shfl(st, q * W, 64);
+    const GuanItemLds itq = guan_item_at(items + q * stride, out_max, in_max);
+    uint64_t totq = 1;
+    for (int d = 0; d < Dq; ++d) totq *= (uint64_t)(itq.s[d] + 1);
+    guan_item_run<JB, 64>(itq, Dq, Jq, ssq, stq, totq, bq, in_max, lane, lane, formula, mode,
+                          Macc, rr, uu, out);
   }
 }
 
@@ -233,10 +317,13 @@ __global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int 
 
 namespace tbg {
 
-size_t guan_lds_bytes(int out_max, int in_max, int mode) {
-  size_t bytes = sizeof(tb::cd) * (size_t)out_max * in_max;
-  if (mode == 1) bytes += sizeof(tb::cd) * (size_t)in_max * 64;
-  bytes += sizeof(int) * ((size_t)in_max * 2 + out_max + 4);
+// Below this batch a launch keeps one item per wave, so a batch of large items still
+// spreads over enough waves.
+constexpr int kGuanPackMinBatch = 16384;
+
+size_t guan_lds_bytes(int out_max, int in_max, int mode, int ipw) {
+  size_t bytes = (mode == 1) ? sizeof(tb::cd) * (size_t)in_max * 64 : 0;
+  bytes += (size_t)ipw * tb::guan_item_stride(out_max, in_max);
   bytes += (size_t)in_max * 64 * 2;
   return bytes;
 }
@@ -244,21 +331,29 @@ size_t guan_lds_bytes(int out_max, int in_max, int mode) {
 int32_t run_guan(const tb::cd *U, int ldu, const int *out_modes, const int *out_mult, int out_max,
                  const int *in_modes, const int *in_mult, int in_max, int batch, int formula,
                  int mode, tb::cd *out, hipStream_t s) {
-  const size_t lds = guan_lds_bytes(out_max, in_max, mode);
-  if (lds > 64 * 1024)
+  const size_t lds1 = guan_lds_bytes(out_max, in_max, mode, 1);
+  if (lds1 > 64 * 1024)
     return tbh::fail(TB_ERANGE, "guan: out_max=%d in_max=%d need %zu bytes of LDS", out_max,
-                     in_max, lds);
-#define TB_GUAN_LAUNCH(JB)                                                                    \
-  hipLaunchKernelGGL((tb::guan_kernel<JB>), dim3(batch), dim3(64), lds, s, U, ldu, out_modes, \
-                     out_mult, out_max, in_modes, in_mult, in_max, formula, mode, out)
-  if (out_max <= 8)
-    TB_GUAN_LAUNCH(8);
+                     in_max, lds1);
+  const size_t lds4 = guan_lds_bytes(out_max, in_max, mode, 4);
+  const bool pack = batch >= kGuanPackMinBatch && out_max <= 16 && lds4 <= 32 * 1024;
+#define TB_GUAN_LAUNCH(JB, IPW, LDS)                                                         \
+  hipLaunchKernelGGL((tb::guan_kernel<JB, IPW>), dim3((batch + IPW - 1) / IPW), dim3(64), LDS, \
+                     s, U, ldu, out_modes, out_mult, out_max, in_modes, in_mult, in_max, batch, \
+                     formula, mode, out)
+  if (pack) {
+    if (out_max <= 8)
+      TB_GUAN_LAUNCH(8, 4, lds4);
+    else
+      TB_GUAN_LAUNCH(16, 4, lds4);
+  } else if (out_max <= 8)
+    TB_GUAN_LAUNCH(8, 1, lds1);
   else if (out_max <= 16)
-    TB_GUAN_LAUNCH(16);
+    TB_GUAN_LAUNCH(16, 1, lds1);
   else if (out_max <= 32)
-    TB_GUAN_LAUNCH(32);
+    TB_GUAN_LAUNCH(32, 1, lds1);
   else
-    TB_GUAN_LAUNCH(48);
+    TB_GUAN_LAUNCH(48, 1, lds1);
 #undef TB_GUAN_LAUNCH
   return tbh::check_hip(hipGetLastError(), "guan_kernel");
 }
