@@ -338,9 +338,10 @@ def test_uniform_loss_m80_n40_4096_samples_device_rng(T, oracle):
 def test_clements_fit_m16_n8_10000_samples(T, oracle):
     """Interferometer fitting at its configured size: a Clements mesh of m = 16 modes (120
     beam splitters) is fitted by Adam, through the permanent gradient kernel, to 10,000
-    samples of n = 8 photons drawn by the GPU sampler from a seeded target mesh; the
-    negative log-likelihood falls by more than 1 nat and ends within 1.5 nats of the
-    target's own."""
+    samples of n = 8 photons drawn by the GPU sampler from a seeded target mesh. 500 Adam
+    steps with the learning rate decayed from 0.1 to 0.001, best of 4 random starts (some
+    starts settle in a basin near 11.8 nats): the negative log-likelihood ends within 0.1
+    nat of the target's own (11.02)."""
     from theboss_amd import ops
     from theboss_amd.interferometer import (clements_unitary, fit_clements,
                                             negative_log_likelihood, sample_index_tensors)
@@ -357,7 +358,8 @@ def test_clements_fit_m16_n8_10000_samples(T, oracle):
     with T.no_grad():
         nll_target = float(negative_log_likelihood(
             target, T.from_numpy(rows).cuda(), T.arange(n, device="cuda"), T.from_numpy(lf).cuda()))
-    _, _, hist = fit_clements(samples, list(range(n)), m, steps=150, lr=0.05, seed=1)
+    _, _, hist = fit_clements(samples, list(range(n)), m, steps=500, lr=0.1, lr_end=0.001,
+                              seed=1, restarts=4)
     print(f"Clements fit: NLL {hist[0]:.3f} -> {hist[-1]:.3f} (target {nll_target:.3f})")
     assert hist[-1] < hist[0] - 1.0
-    assert hist[-1] < nll_target + 1.5
+    assert hist[-1] < nll_target + 0.1

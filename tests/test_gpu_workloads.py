@@ -226,6 +226,14 @@ def test_clements_fit_reduces_nll(T, oracle):
     samples = out.cpu().numpy()
     _, _, hist = fit_clements(samples, list(range(n)), m, steps=60, lr=0.05, seed=1)
     assert hist[-1] < hist[0] - 0.05, hist[::10]
+    # restarts: the run with the lowest final NLL among the starts seed, seed + 1, ...
+    ends = [fit_clements(samples, list(range(n)), m, steps=20, lr=0.1, lr_end=0.01, seed=s)[2][-1]
+            for s in (4, 5, 6)]
+    th3, ph3, h3 = fit_clements(samples, list(range(n)), m, steps=20, lr=0.1, lr_end=0.01, seed=4,
+                                restarts=3)
+    assert len(h3) == 20 and abs(h3[-1] - min(ends)) <= 1e-9 * abs(min(ends)), (h3[-1], ends)
+    with pytest.raises(ValueError):
+        fit_clements(samples, list(range(n)), m, steps=1, restarts=0)
 
 
 # ------------------------------------------------------------- exact distributions

@@ -70,30 +70,46 @@ def negative_log_likelihood(U, rows, cols, logfact):
 
 
 def fit_clements(samples: np.ndarray, input_modes: Sequence[int], m: int, steps: int = 200,
-                 lr: float = 0.05, seed: int = 0, init=None):
-    """Adam on the NLL; returns (theta, phi, history of NLL values)."""
+                 lr: float = 0.05, seed: int = 0, init=None, lr_end: float = None,
+                 restarts: int = 1):
+    """Adam on the NLL; returns (theta, phi, history of NLL values). With lr_end the learning
+    rate falls from lr to lr_end along a half cosine over the steps (large early steps cross
+    the rugged start of the landscape, small late ones settle into the basin). The
+    landscape has poor local minima that some starting points cannot leave: restarts > 1
+    runs the fit from the random starts seed, seed + 1, ... and returns the run with the
+    lowest final NLL (with init, every restart starts there and restarts is pointless)."""
     T = D.require_gpu()
     dev = D.device()
-    nunits = m * (m - 1) // 2
-    g = T.Generator(device="cpu").manual_seed(seed)
-    if init is None:
-        theta = (T.rand(nunits, generator=g, dtype=T.float64) * math.pi / 2).to(dev)
-        phi = (T.rand(nunits, generator=g, dtype=T.float64) * 2 * math.pi).to(dev)
-    else:
-        theta, phi = (x.detach().clone().to(dev) for x in init)
-    theta.requires_grad_(True)
-    phi.requires_grad_(True)
+    if restarts < 1:
+        raise ValueError(f"restarts={restarts} < 1")
     rows, logfact = sample_index_tensors(samples, input_modes)
     rows_t = T.from_numpy(rows).to(dev)
     cols_t = T.tensor(list(input_modes), dtype=T.long, device=dev)
     lf_t = T.from_numpy(logfact).to(dev)
-    opt = T.optim.Adam([theta, phi], lr=lr)
-    hist = []
-    for _ in range(steps):
-        opt.zero_grad()
-        U = clements_unitary(theta, phi, m)
-        loss = negative_log_likelihood(U, rows_t, cols_t, lf_t)
-        loss.backward()
-        opt.step()
-        hist.append(float(loss.detach().cpu()))
-    return theta.detach(), phi.detach(), hist
+    nunits = m * (m - 1) // 2
+    best = None
+    for r in range(restarts):
+        g = T.Generator(device="cpu").manual_seed(seed + r)
+        if init is None:
+            theta = (T.rand(nunits, generator=g, dtype=T.float64) * math.pi / 2).to(dev)
+            phi = (T.rand(nunits, generator=g, dtype=T.float64) * 2 * math.pi).to(dev)
+        else:
+            theta, phi = (x.detach().clone().to(dev) for x in init)
+        theta.requires_grad_(True)
+        phi.requires_grad_(True)
+        opt = T.optim.Adam([theta, phi], lr=lr)
+        hist = []
+        for step in range(steps):
+            if lr_end is not None:
+                frac = step / max(steps - 1, 1)
+                for grp in opt.param_groups:
+                    grp["lr"] = lr_end + 0.5 * (lr - lr_end) * (1.0 + math.cos(math.pi * frac))
+            opt.zero_grad()
+            U = clements_unitary(theta, phi, m)
+            loss = negative_log_likelihood(U, rows_t, cols_t, lf_t)
+            loss.backward()
+            opt.step()
+            hist.append(float(loss.detach().cpu()))
+        if best is None or hist[-1] < best[2][-1]:
+            best = (theta.detach(), phi.detach(), hist)
+    return best
