@@ -10,6 +10,7 @@ samples one after the other and computes each permanent with a separate calculat
 here the samples advance together, and all permanents of the layers a round needs
 (m x C(n, k) per new prefix) go to the GPU in one batched launch.
 """
+import itertools
 from collections import defaultdict
 from math import comb, factorial
 from typing import Dict, List, Optional, Sequence, Tuple
@@ -23,12 +24,9 @@ from .simulation_strategy_interface import SimulationStrategyInterface
 
 def input_substates(state: Sequence[int]) -> List[Tuple[int, ...]]:
     """All substates r <= s in the reference's order
-    (generalized_cliffords_simulation_strategy.py:94-121: first mode outermost)."""
-    if len(state) < 1:
-        return [tuple()]
-    first, rest = int(state[0]), list(state[1:])
-    smaller = input_substates(rest)
-    return [(i,) + sub for i in range(first + 1) for sub in smaller]
+    (generalized_cliffords_simulation_strategy.py:94-121: first mode outermost, which is
+    the order of a Cartesian product over the modes)."""
+    return list(itertools.product(*[range(int(c) + 1) for c in state]))
 
 
 class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
@@ -114,18 +112,33 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
             self._sample_from_latest_pmf()
 
     def _weights(self, substates) -> np.ndarray:
-        """generalized_cliffords_simulation_strategy.py:173-205, normalised (:155)."""
+        """generalized_cliffords_simulation_strategy.py:173-205, normalised (:155): for
+        each substate s with remainder kv = input - s of l particles,
+        l! (n - l)! / n! prod_i C(input_i, kv_i), the binomials multiplied in mode order as
+        the reference does, a column of substates at a time."""
         n = self.number_of_input_photons
-        w = []
-        for s in substates:
-            kv = [self.input_state[i] - s[i] for i in range(len(s))]
-            l = sum(kv)
-            val = factorial(l) * factorial(n - l) / factorial(n)
-            for i in range(len(self.input_state)):
-                val *= comb(self.input_state[i], kv[i])
-            w.append(val)
-        w = np.asarray(w, dtype=np.float64)
+        inp = np.asarray(self.input_state, dtype=np.int64)
+        S = np.asarray(substates, dtype=np.int64).reshape(len(substates), len(inp))
+        kv = inp[None, :] - S
+        base = np.array([factorial(l) * factorial(n - l) / factorial(n) for l in range(n + 1)])
+        w = base[kv.sum(axis=1)]
+        top = int(inp.max()) if inp.size else 0
+        binom = np.array([[float(comb(a, b)) for b in range(top + 1)] for a in range(top + 1)])
+        for i in range(len(inp)):
+            w = w * binom[inp[i], kv[:, i]]
         return w / w.sum()
+
+    def _norms(self, S: np.ndarray, k: int) -> np.ndarray:
+        """prod_i s_i! k! per substate row of S (k particles each), as the float of the
+        exact integer product times k! (the product is at most k!, exact in int64 for
+        k <= 20; Python integers above)."""
+        if k <= 20:
+            fact = np.array([factorial(c) for c in range(k + 1)], dtype=np.int64)
+            prods = np.prod(fact[S], axis=1).astype(np.float64)
+        else:
+            prods = np.array([float(np.prod([factorial(c) for c in row], dtype=object))
+                              for row in S.tolist()])
+        return prods * factorial(k)
 
     def _layer_constants(self, k: int):
         """Per particle number k: the substates as an array, their normalised weights and
@@ -133,9 +146,7 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
         if k not in self._layer_cache:
             substates = self._labeled_states[k]
             S = np.asarray(substates, dtype=np.int64).reshape(len(substates), -1)
-            norm = np.array([float(np.prod([factorial(c) for c in s])) * factorial(sum(s))
-                             for s in substates])
-            self._layer_cache[k] = (S, self._weights(substates), norm)
+            self._layer_cache[k] = (S, self._weights(substates), self._norms(S, k))
         return self._layer_cache[k]
 
     def _calculate_new_layer_of_pmfs(self) -> None:

@@ -34,14 +34,18 @@ def compact(state: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
 
 def compact_batch(states: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
     """Row-wise compact for a B x m stack: occupied modes first (in mode order), padded
-    with multiplicity 0. Returns (modes, multiplicities) as B x W int32, W >= 1."""
+    with mode 0 of multiplicity 0. Returns (modes, multiplicities) as B x W int32, W >= 1."""
     states = np.asarray(states)
     nz = states > 0
     cnt = nz.sum(axis=1)
     W = max(1, int(cnt.max()) if cnt.size else 1)
-    order = np.argsort(~nz, axis=1, kind="stable")[:, :W]
-    mult = np.take_along_axis(states, order, axis=1) * (np.arange(W)[None, :] < cnt[:, None])
-    return order.astype(np.int32), mult.astype(np.int32)
+    rows, cols = np.nonzero(nz)  # row-major: each row's occupied modes in mode order
+    slot = np.arange(rows.size) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+    modes = np.zeros((states.shape[0], W), dtype=np.int32)
+    mult = np.zeros((states.shape[0], W), dtype=np.int32)
+    modes[rows, slot] = cols
+    mult[rows, slot] = states[rows, cols]
+    return modes, mult
 
 
 def effective_matrix(matrix, input_state, output_state) -> np.ndarray:
