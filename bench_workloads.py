@@ -133,12 +133,20 @@ def glynn_flops(n):
     return (8 * n - 4) * 2 ** (n - 1)
 
 
+def glynn_instructions_per_code(n):
+    """FP64 instructions per Gray code of the kernel that runs n (FP64, generic matrices).
+    Paired walk (14 <= n <= 39): per pair of codes one row flip (2n) and two products
+    of two folded chains each, joined in the accumulation (2 (4n - 2)): 5n - 2 per code.
+    Plain walk: 2n column adds and n - 1 complex multiplies of 4 instructions, the last
+    one fused into the accumulation: 6n - 4."""
+    return 5 * n - 2 if 14 <= n <= 39 else 6 * n - 4
+
+
 def glynn_issue_ceiling(n, clock_ghz):
-    """Fraction of the FP64 peak (2.4 GHz, every instruction an FMA) that the Glynn step can
-    reach when its instructions issue back to back: 8n - 4 flops per step against 2n
-    column adds and n - 1 complex multiplies of 4 instructions (the last one fused into the
-    accumulation), i.e. 6n - 4 instructions of 2 peak flops each."""
-    return (8 * n - 4) / (2 * (6 * n - 4)) * clock_ghz / 2.4
+    """Fraction of the FP64 peak (2.4 GHz, every instruction an FMA) that the Glynn kernel
+    can reach when its instructions issue back to back: the 8n - 4 flops of a Gray code
+    against its instructions, 2 peak flops each, at the measured clock."""
+    return (8 * n - 4) / (2 * glynn_instructions_per_code(n)) * clock_ghz / 2.4
 
 
 def _glynn_blocks(checker, n, B=16):

@@ -113,11 +113,16 @@ static inline dd dd_add(dd a, dd b) {
  * standard reflected Gray code flips bit ctz(step) at step = 1..2^(n-1)-1, which
  * visits the same 2^(n-1) delta vectors, so the sum is the same up to rounding.
  * mode 0: the reference's arithmetic (serial FP64 accumulation, incremental sums).
- * mode 1: oracle-grade: column sums re-seeded from delta every 4096 steps and the
- *         alternating sum accumulated in double-double. Used for n >= 20.
+ * mode 1: column sums re-seeded from delta every 4096 steps and the alternating sum
+ *         accumulated in double-double (FP64 sums and products: ~1e-13 relative at
+ *         n = 18, less accurate as n and the cancellation of the sum grow).
+ * mode 2: the same walk in 80-bit long double throughout (glynn_ld below).
  * --------------------------------------------------------------------------------- */
+static cplx glynn_ld(const cplx *M, int n);
+
 cplx orc_perm_glynn_matrix(const cplx *M, int n, int mode) {
   if (n == 0) return 1.0;
+  if (mode == 2) return glynn_ld(M, n);
   int delta[ORC_MAX_N];
   cplx sums[ORC_MAX_N];
   for (int i = 0; i < n; ++i) delta[i] = 1;
@@ -173,10 +178,107 @@ cplx orc_perm_glynn(const cplx *U, int m, const int *s, const int *t, int mode) 
   return r;
 }
 
-/* Batch of square matrices, OpenMP over the batch (CPU baseline leg). */
+/* ---------------------------------------------------------------------------------
+ * Mode 2 (reference grade for n >= 24, where mode 1's FP64 column sums and products
+ * limit it to ~1e-13 .. 1e-11 relative): the same Gray walk in 80-bit long double.
+ * Column sums are seeded exactly from delta at the start of every chunk of 4096 steps
+ * and updated by +-2 M[i][j]; products form a balanced tree; a chunk's alternating sum
+ * is compensated (Neumaier) and the chunk sums are added in chunk order, so the result
+ * does not depend on the thread count. OpenMP over the chunks of one matrix.
+ * --------------------------------------------------------------------------------- */
+typedef long double ldbl;
+
+static inline void ld_neumaier(ldbl *s, ldbl *c, ldbl x) {
+  const ldbl t = *s + x;
+  if (fabsl(*s) >= fabsl(x))
+    *c += (*s - t) + x;
+  else
+    *c += (x - t) + *s;
+  *s = t;
+}
+
+static inline void ld_tree_prod(const ldbl *re, const ldbl *im, int n, ldbl *pr, ldbl *pi) {
+  ldbl tr[ORC_MAX_N], ti[ORC_MAX_N];
+  for (int j = 0; j < n; ++j) { tr[j] = re[j]; ti[j] = im[j]; }
+  int m = n;
+  while (m > 1) {
+    const int h = m / 2;
+    for (int k = 0; k < h; ++k) {
+      const ldbl ar = tr[2 * k], ai = ti[2 * k], br = tr[2 * k + 1], bi = ti[2 * k + 1];
+      tr[k] = ar * br - ai * bi;
+      ti[k] = ar * bi + ai * br;
+    }
+    if (m & 1) { tr[h] = tr[m - 1]; ti[h] = ti[m - 1]; }
+    m = h + (m & 1);
+  }
+  *pr = tr[0];
+  *pi = ti[0];
+}
+
+static cplx glynn_ld(const cplx *M, int n) {
+  if (n == 0) return 1.0;
+  const int nb = n - 1;
+  const int cbits = nb < 12 ? nb : 12;
+  const uint64_t nchunks = 1ull << (nb - cbits);
+  ldbl *cre = (ldbl *)malloc(sizeof(ldbl) * nchunks), *cim = (ldbl *)malloc(sizeof(ldbl) * nchunks);
+  ldbl Mr[ORC_MAX_N * ORC_MAX_N], Mi[ORC_MAX_N * ORC_MAX_N];
+  for (int e = 0; e < n * n; ++e) { Mr[e] = creal(M[e]); Mi[e] = cimag(M[e]); }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+  for (uint64_t q = 0; q < nchunks; ++q) {
+    const uint64_t t0 = q << cbits;
+    const uint64_t g0 = t0 ^ (t0 >> 1);
+    int delta[ORC_MAX_N];
+    for (int i = 0; i < n; ++i) delta[i] = (i < nb && ((g0 >> i) & 1ull)) ? -1 : 1;
+    ldbl sr[ORC_MAX_N], si[ORC_MAX_N];
+    for (int j = 0; j < n; ++j) {
+      ldbl a = 0, b = 0;
+      for (int i = 0; i < n; ++i) { a += delta[i] * Mr[i * n + j]; b += delta[i] * Mi[i * n + j]; }
+      sr[j] = a;
+      si[j] = b;
+    }
+    ldbl s_re = 0, c_re = 0, s_im = 0, c_im = 0, pr, pi;
+    ld_tree_prod(sr, si, n, &pr, &pi);
+    const ldbl sg0 = (t0 & 1ull) ? -1.0L : 1.0L;  /* (-1)^popcount(gray(t)) = (-1)^t */
+    ld_neumaier(&s_re, &c_re, sg0 * pr);
+    ld_neumaier(&s_im, &c_im, sg0 * pi);
+    for (uint64_t l = 1; l < (1ull << cbits); ++l) {
+      const uint64_t t = t0 + l;
+      const int i = __builtin_ctzll(t);
+      delta[i] = -delta[i];
+      const ldbl d2 = 2.0L * delta[i];
+      for (int j = 0; j < n; ++j) { sr[j] += d2 * Mr[i * n + j]; si[j] += d2 * Mi[i * n + j]; }
+      ld_tree_prod(sr, si, n, &pr, &pi);
+      const ldbl sg = (t & 1ull) ? -1.0L : 1.0L;
+      ld_neumaier(&s_re, &c_re, sg * pr);
+      ld_neumaier(&s_im, &c_im, sg * pi);
+    }
+    cre[q] = s_re + c_re;
+    cim[q] = s_im + c_im;
+  }
+  ldbl s_re = 0, c_re = 0, s_im = 0, c_im = 0;
+  for (uint64_t q = 0; q < nchunks; ++q) {
+    ld_neumaier(&s_re, &c_re, cre[q]);
+    ld_neumaier(&s_im, &c_im, cim[q]);
+  }
+  free(cre);
+  free(cim);
+  const ldbl scale = ldexpl(1.0L, -nb);
+  return (double)((s_re + c_re) * scale) + I * (double)((s_im + c_im) * scale);
+}
+
+/* Batch of square matrices, OpenMP over the batch (CPU baseline leg); mode 2 runs the
+ * matrices one after another, each parallel over its chunks. */
 void orc_perm_glynn_batch(const cplx *A, int n, int batch, int mode, int threads, cplx *out) {
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
+#endif
+  if (mode == 2) {
+    for (int b = 0; b < batch; ++b) out[b] = glynn_ld(A + (size_t)b * n * n, n);
+    return;
+  }
+#ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1)
 #endif
   for (int b = 0; b < batch; ++b) out[b] = orc_perm_glynn_matrix(A + (size_t)b * n * n, n, mode);

@@ -89,6 +89,44 @@ def test_glynn_closed_forms_large(oracle, n):
     assert rel(oracle.perm_glynn_matrix(J - np.eye(n), 1), oracle.derangements(n)) < 1e-12
 
 
+def _glynn_clongdouble(A):
+    """Independent 80-bit restatement of the Glynn walk: one sequential reflected-Gray walk
+    in numpy clongdouble (no chunks, no re-seeding), plain summation."""
+    M = np.asarray(A).astype(np.clongdouble)
+    n = M.shape[0]
+    delta = np.ones(n, dtype=np.longdouble)
+    h = M.sum(axis=0)
+    acc = np.prod(h)
+    for t in range(1, 1 << (n - 1)):
+        i = (t & -t).bit_length() - 1
+        delta[i] = -delta[i]
+        h = h + 2 * delta[i] * M[i]
+        acc = acc + (-1 if t & 1 else 1) * np.prod(h)
+    return complex(acc / np.longdouble(2) ** (n - 1))
+
+
+def test_glynn_80bit_mode(oracle):
+    """Mode 2 (80-bit walk, the reference grade of the full-size golden values): brute force
+    up to n = 8, closed forms (perm J = n!, perm(J - I) = !n, rank one n! prod u prod v)
+    at n = 12..20, and the independent clongdouble walk above at n = 10..13; mode 1 is
+    within its FP64 error of mode 2."""
+    rng = np.random.default_rng(11)
+    for n in range(1, 9):
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        assert rel(oracle.perm_glynn_batch(A[None], 2)[0], oracle.perm_classic_matrix(A)) < 1e-14
+    for n in (12, 16, 20):
+        u, v = np.exp(2j * np.pi * rng.random(n)), np.exp(2j * np.pi * rng.random(n))
+        J = np.ones((n, n), dtype=complex)
+        got = oracle.perm_glynn_batch(np.stack([np.outer(u, v), J, J - np.eye(n)]), 2)
+        assert rel(got[0], math.factorial(n) * np.prod(u) * np.prod(v)) < 1e-15
+        assert got[1] == math.factorial(n) and got[2] == oracle.derangements(n)
+    for n in (10, 13):
+        A = oracle.haar_unitary(2 * n, 700 + n)[:n, :n]
+        ld = _glynn_clongdouble(A)
+        assert rel(oracle.perm_glynn_batch(A[None], 2)[0], ld) < 1e-15
+        assert rel(oracle.perm_glynn_matrix(A, 1), ld) < 1e-12
+
+
 def test_glynn_invariances(oracle):
     n = 9
     rng = np.random.default_rng(3)

@@ -201,6 +201,70 @@ __device__ __forceinline__ void flip_bcast(double (&hr)[N], double (&hi)[N], con
   }
 }
 
+// ------------------------------------------------------------------------------------
+// Paired walks. Two Gray codes that differ only in one row (particle) r have column sums
+// c_j (u_j + 1) and c_j (u_j - 1) once every other row is divided by row r
+// (u_j = sum_{i != r} delta_i M[i][j] / M[r][j]), so one update of u serves both codes
+// and each factor (u_j +- 1) enters a running product x as x u_j +- x: four FMAs with
+// the +-1 folded into their addends.
+// ------------------------------------------------------------------------------------
+
+// prod_{j in [LO, HI)} (u_j + PM), PM = +-1: the first factor formed by one add, each
+// further one folded into x u_j + PM x.
+template <int PM, int LO, int HI, int N>
+__device__ __forceinline__ cd fold_chain(const double (&ur)[N], const double (&ui)[N]) {
+  double xr = ur[LO] + (double)PM, xi = ui[LO];
+#pragma unroll
+  for (int j = LO + 1; j < HI; ++j) {
+    const double nr = fma(-xi, ui[j], fma(xr, ur[j], (double)PM * xr));
+    const double ni = fma(xi, ur[j], fma(xr, ui[j], (double)PM * xi));
+    xr = nr;
+    xi = ni;
+  }
+  return {xr, xi};
+}
+
+// X += SIGN prod(u + 1), Y += SIGN prod(u - 1); each product is two chains of N/2
+// factors joined by a multiply fused into the accumulation.
+template <int SIGN, int N>
+__device__ __forceinline__ void pair_acc2(double &xr, double &xi, double &yr, double &yi,
+                                          const double (&ur)[N], const double (&ui)[N]) {
+  constexpr int H = N / 2;
+  cmul_acc<SIGN>(xr, xi, fold_chain<1, 0, H>(ur, ui), fold_chain<1, H, N>(ur, ui));
+  cmul_acc<SIGN>(yr, yi, fold_chain<-1, 0, H>(ur, ui), fold_chain<-1, H, N>(ur, ui));
+}
+
+// 1 / c for c != 0, with c scaled by a power of two so that |c|^2 stays in range.
+__device__ __forceinline__ cd cinv_scaled(cd c) {
+  const int e = ilogb(fmax(fabs(c.re), fabs(c.im)));
+  const double cr = ldexp(c.re, -e), ci = ldexp(c.im, -e);
+  const double d = fma(cr, cr, ci * ci);
+  return {ldexp(cr / d, -e), ldexp(-ci / d, -e)};
+}
+
+// A pair row must have every entry above 2^-16 of the matrix's largest entry (squared
+// magnitudes compared): far from zero, its quotients stay in range and well rounded.
+constexpr double kPairMinRel2 = 0x1p-32;
+
+// c = prod_j v[j] * 2^-J (J values) as a mantissa and a power of two, renormalised after
+// every factor so that no partial product over- or underflows.
+__device__ __forceinline__ void prod_half_scaled(const cd *v, int J, cd &mant, int &e2) {
+  cd k = {1.0, 0.0};
+  int ke = -J;
+  for (int j = 0; j < J; ++j) {
+    k = cmul(k, v[j]);
+    const double a = fmax(fabs(k.re), fabs(k.im));
+    if (a > 0.0) {
+      const int e = ilogb(a);
+      k.re = ldexp(k.re, -e);
+      k.im = ldexp(k.im, -e);
+      ke += e;
+    }
+  }
+  mant = k;
+  e2 = ke;
+}
+
 // 1.0 in a VGPR the compiler cannot fold away (second operand of fixed-direction flips).
 __device__ __forceinline__ double vgpr_one() {
   double one = 1.0;

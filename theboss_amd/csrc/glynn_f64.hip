@@ -6,20 +6,24 @@
 
 namespace tbgl {
 int32_t glynn_dispatch_f64(int n, const tb::cpx<double> *A, int64_t mstride, int batch,
-                           tb::cd *out, tb::cd *partial, int c_pref, hipStream_t s) {
-  return dispatch_glynn<double>(n, A, mstride, batch, out, partial, c_pref, s);
+                           tb::cd *out, tb::cd *partial, int *handled, int c_pref, hipStream_t s) {
+  return dispatch_glynn<double>(n, A, mstride, batch, out, partial, handled, c_pref, s);
 }
 }  // namespace tbgl
 
 namespace {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// The "handled" flags of the paired walk (one int per matrix) ahead of the partials.
+size_t tbgl_flag_bytes(int batch) { return align256((size_t)batch * sizeof(int)); }
+
 size_t tbgl_partial_bytes(int n, int batch, int c_pref) {
   if (n < tbgl::kWideMinN) return 0;
-  // large enough for the plan of either precision (their static bits may differ)
+  // large enough for the plan of either precision (their lower bounds on C may differ;
+  // a larger bound only shrinks wpm)
   const int wpm = std::max(tbgl::glynn_plan(n, batch, c_pref, 2).wpm,
                            tbgl::glynn_plan(n, batch, c_pref, tbgl::glynn_u_max(n)).wpm);
-  return align256((size_t)batch * wpm * sizeof(tb::cd));
+  return tbgl_flag_bytes(batch) + align256((size_t)batch * wpm * sizeof(tb::cd));
 }
 
 // The precision a call runs in: FP32 column sums only up to TB_FP32_COLSUM_MAX_N unless
@@ -74,10 +78,11 @@ extern "C" int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t bat
     return tbh::check_hip(hipGetLastError(), "copy_scalar_kernel");
   }
   char *ws = (char *)workspace;
-  tb::cd *partial = (tb::cd *)ws;
+  int *handled = (int *)ws;
+  tb::cd *partial = ws ? (tb::cd *)(ws + tbgl_flag_bytes(batch)) : nullptr;
   if (!runs_fp32(n, precision))
     return tbgl::glynn_dispatch_f64(n, (const tb::cpx<double> *)A, matrix_stride, batch, o,
-                                    partial, c_pref, s);
+                                    partial, handled, c_pref, s);
   // FP32 column sums: convert once into the tail of the workspace.
   float *A32 = (float *)(ws + tbgl_partial_bytes(n, batch, c_pref));
   const int64_t total = (int64_t)batch * n * n;

@@ -219,14 +219,17 @@ __device__ __forceinline__ void glynn_lane(P M, int C, uint64_t q0, uint64_t qst
 // WIDE: the 4 waves of a workgroup share one matrix, staged once into LDS; every flip is
 // a broadcast LDS read. Grid: batch * (wpm/4) workgroups; wave wq of matrix b covers
 // chunks q = (wq + it*wpm)*64 + lane. Strides count complex elements.
+// `handled` (may be null): matrices whose partials the paired kernel already wrote.
 template <int N, int U, int R0, typename F>
 __global__ __launch_bounds__(256) TB_WIDE_ATTR(N) void glynn_wide_kernel(const cpx<F> *__restrict__ A,
                                                          int64_t mstride, int C, int wpm, int iters,
-                                                         cd *__restrict__ partial) {
+                                                         cd *__restrict__ partial,
+                                                         const int *__restrict__ handled) {
   __shared__ cpx<F> Ms[N * N];
   const int lane = threadIdx.x & 63;
   const int wgs_per_matrix = wpm >> 2;
   const int b = blockIdx.x / wgs_per_matrix;
+  if (handled != nullptr && handled[b] != 0) return;
   const int wq = (blockIdx.x - b * wgs_per_matrix) * 4 + (threadIdx.x >> 6);
   const cpx<F> *src = A + (size_t)b * mstride;
   for (int e = threadIdx.x; e < N * N; e += 256) Ms[e] = src[e];
@@ -244,6 +247,222 @@ __global__ __launch_bounds__(256) TB_WIDE_ATTR(N) void glynn_wide_kernel(const c
   are = wave_sum(are);
   aim = wave_sum(aim);
   if (lane == 0) partial[(size_t)b * wpm + wq] = {are, aim};
+}
+
+// ------------------------------------------------------------------------------------
+// Paired Gray walk (FP64 WIDE). Gray codes 2t and 2t+1 differ only in row 0. With row
+// N-1 fixed (delta = +1) and u_j = sum_{i >= 1} delta_i M[i][j] / M[0][j], their half
+// column sums are h_j = (M[0][j] / 2)(u_j + 1) and (M[0][j] / 2)(u_j - 1), so
+//   perm = 2 K sum_t (-1)^t [prod_j (u_j + 1) - prod_j (u_j - 1)],  K = prod_j M[0][j] / 2,
+// over the 2^(N-2) Gray ranks t of rows 1..N-2. A flip of row i adds +-2 M[i][j] / M[0][j]
+// to every u_j (one update for both codes), and a factor (u_j +- 1) enters a running
+// product x as x u_j +- x: four FMAs with the +-1 folded into their addends. A pair of
+// codes costs 2N flip instructions and 2 (4N - 2) product instructions (each product two
+// chains of N/2 factors, joined by a multiply fused into the accumulation) against
+// 2 (6N - 4) for two plain steps. The row whose smallest entry is largest becomes row 0
+// (a row permutation leaves the permanent unchanged); a matrix none of whose rows has all
+// entries above 2^-16 of its largest entry (zeros included) is left to the plain kernel.
+// ------------------------------------------------------------------------------------
+
+// acc += SIGN * (prod (u + 1) - prod (u - 1)).
+template <int SIGN, int N>
+__device__ __forceinline__ void pair_acc(double &ar, double &ai, const double (&ur)[N],
+                                         const double (&ui)[N]) {
+  constexpr int H = N / 2;
+  const cd ap = fold_chain<1, 0, H>(ur, ui), bp = fold_chain<1, H, N>(ur, ui);
+  const cd am = fold_chain<-1, 0, H>(ur, ui), bm = fold_chain<-1, H, N>(ur, ui);
+  cmul_acc<SIGN>(ar, ai, ap, bp);
+  cmul_acc<-SIGN>(ar, ai, am, bm);
+}
+
+// u += sg * row i + 1 of S2 (rows 1..B in broadcast form, the others from LDS).
+template <int N, int MODE, int B>
+__device__ __forceinline__ void pair_flip(double (&ur)[N], double (&ui)[N], const BcastRows<N, B> &bs,
+                                          const cpx<double> *S2, int i, double sg) {
+  if constexpr (B > 0) {
+    if (i < B) {  // compile-time at the static flip sites, uniform at block starts
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+        if (u == i) flip_bcast<0, N, MODE>(ur, ui, bs.r[u], MODE == 0 ? sg : bs.one);
+      return;
+    }
+  }
+  flip_mem<N, MODE, double>(ur, ui, S2 + (size_t)(i + 1) * N, sg);
+}
+
+// One lane walks `iters` chunks q = q0 + it*qstride of 2^C super-steps (Gray ranks of rows
+// 1..N-2) each; S2 holds 2 M[i][j] / M[0][j] in rows 1..N-1. Same chunk, block and
+// direction structure as glynn_lane with row i + 1 at Gray bit i.
+template <int N, int U, int B>
+__device__ __forceinline__ void glynn_pair_lane(const cpx<double> *S2, int C, uint64_t q0,
+                                                uint64_t qstride, int iters, double &acc_re,
+                                                double &acc_im) {
+  constexpr int NB = N - 2;  // Gray bits
+  const bool topbit_fixed = (C == NB);
+  BcastRows<N, B> bs;
+  if constexpr (B > 0) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int u = 0; u < B; ++u)
+      load_bcast(bs.r[u], (const double *)(S2 + (size_t)(u + 1) * N), lane);
+    bs.one = vgpr_one();
+    asm volatile("s_nop 4");  // VALU writes of the store before its first DPP read
+  }
+  for (int it = 0; it < iters; ++it) {
+    const uint64_t q = q0 + (uint64_t)it * qstride;
+    const uint64_t t0 = q << C;
+    const uint64_t g0 = t0 ^ (t0 >> 1);
+    double ur[N], ui[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const cpx<double> v = S2[(size_t)(N - 1) * N + j];
+      ur[j] = v.re;
+      ui[j] = v.im;
+    }
+#pragma unroll 1
+    for (int i = 0; i < NB; ++i) {
+      const double sg = ((g0 >> i) & 1ull) ? -1.0 : 1.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const cpx<double> v = S2[(size_t)(i + 1) * N + j];
+        ur[j] = fma(sg, v.re, ur[j]);
+        ui[j] = fma(sg, v.im, ui[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      ur[j] *= 0.5;
+      ui[j] *= 0.5;
+    }
+    const bool q_odd = (q & 1ull) != 0;
+    double cre = 0.0, cim = 0.0;
+    pair_acc<1>(cre, cim, ur, ui);
+    const int nblocks = 1 << (C - U);
+    auto static_steps = [&](int blk) {
+#pragma unroll
+      for (int j = 1; j < (1 << U); ++j) {
+        const int i = __builtin_ctz(j);
+        if (i + 1 < U) {
+          if ((j >> (i + 1)) & 1)
+            pair_flip<N, 1, B>(ur, ui, bs, S2, i, 1.0);
+          else
+            pair_flip<N, -1, B>(ur, ui, bs, S2, i, -1.0);
+        } else {
+          bool add;
+          if (U < C)
+            add = (blk & 1) != 0;
+          else
+            add = topbit_fixed ? false : q_odd;
+          pair_flip<N, 0, B>(ur, ui, bs, S2, i, add ? 1.0 : -1.0);
+        }
+        if (j & 1)
+          pair_acc<-1>(cre, cim, ur, ui);
+        else
+          pair_acc<1>(cre, cim, ur, ui);
+        TB_STEP_FENCE();
+      }
+    };
+    static_steps(0);
+    for (int blk = 1; blk < nblocks; ++blk) {
+      const int i = U + __builtin_ctz(blk);
+      bool add;
+      if (i + 1 < C)
+        add = ((blk >> (i + 1 - U)) & 1) != 0;
+      else
+        add = topbit_fixed ? false : q_odd;
+      if constexpr (B > U) {
+        if (i < B)
+          pair_flip<N, 0, B>(ur, ui, bs, S2, i, add ? 1.0 : -1.0);
+        else
+          flip_mem<N, 0, double>(ur, ui, S2 + (size_t)(i + 1) * N, add ? 1.0 : -1.0);
+      } else {
+        flip_mem<N, 0, double>(ur, ui, S2 + (size_t)(i + 1) * N, add ? 1.0 : -1.0);
+      }
+      pair_acc<1>(cre, cim, ur, ui);
+      TB_STEP_FENCE();
+      static_steps(blk);
+    }
+    acc_re += cre;
+    acc_im += cim;
+  }
+}
+
+// PAIRED WIDE kernel: grid, chunks and partial layout of glynn_wide_kernel with C Gray
+// bits per chunk of codes (C - 1 super-step bits). Writes handled[b] (1: this kernel wrote
+// matrix b's partials; 0: left to the plain kernel).
+template <int N, int U, int B>
+__global__ __launch_bounds__(256) TB_WIDE_ATTR(N) void glynn_pair_kernel(
+    const cpx<double> *__restrict__ A, int64_t mstride, int C, int wpm, int iters,
+    cd *__restrict__ partial, int *__restrict__ handled) {
+  __shared__ cpx<double> Ms[N * N];
+  __shared__ cd inv0[N];
+  __shared__ double rmin[N], rmax[N];
+  __shared__ int pick, Ke;
+  __shared__ cd Km;
+  const int lane = threadIdx.x & 63;
+  const int wgs_per_matrix = wpm >> 2;
+  const int b = blockIdx.x / wgs_per_matrix;
+  const int wg = blockIdx.x - b * wgs_per_matrix;
+  const int wq = wg * 4 + (threadIdx.x >> 6);
+  const cpx<double> *src = A + (size_t)b * mstride;
+  for (int e = threadIdx.x; e < N * N; e += 256) Ms[e] = src[e];
+  __syncthreads();
+  if (threadIdx.x < N) {
+    double lo = INFINITY, hi = 0.0;
+    for (int j = 0; j < N; ++j) {
+      const cpx<double> v = Ms[threadIdx.x * N + j];
+      const double a = fma(v.re, v.re, v.im * v.im);
+      lo = fmin(lo, a);
+      hi = fmax(hi, a);
+    }
+    rmin[threadIdx.x] = lo;
+    rmax[threadIdx.x] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int best = 0;
+    double mx = 0.0;
+    for (int i = 0; i < N; ++i) {
+      if (rmin[i] > rmin[best]) best = i;
+      mx = fmax(mx, rmax[i]);
+    }
+    const bool ok = mx > 0.0 && mx <= 1.7976931348623157e308 && rmin[best] >= kPairMinRel2 * mx;
+    pick = ok ? best : -1;
+    if (wg == 0) handled[b] = ok ? 1 : 0;
+  }
+  __syncthreads();
+  const int r = pick;
+  if (r < 0) return;  // uniform over the workgroup
+  if (r != 0 && threadIdx.x < N) {
+    const cpx<double> t = Ms[threadIdx.x];
+    Ms[threadIdx.x] = Ms[r * N + threadIdx.x];
+    Ms[r * N + threadIdx.x] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < N) inv0[threadIdx.x] = cinv_scaled(cd{Ms[threadIdx.x].re, Ms[threadIdx.x].im});
+  if (threadIdx.x == 64) {  // K = prod_j M[0][j] / 2 as mantissa x 2^Ke
+    cd k;
+    int ke;
+    prod_half_scaled((const cd *)Ms, N, k, ke);
+    Km = k;
+    Ke = ke;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < (N - 1) * N; e += 256) {
+    const int i = 1 + e / N, j = e - (i - 1) * N;
+    const cd v = cmul(cd{Ms[i * N + j].re, Ms[i * N + j].im}, inv0[j]);
+    Ms[i * N + j] = {2.0 * v.re, 2.0 * v.im};
+  }
+  __syncthreads();
+  double are = 0.0, aim = 0.0;
+  glynn_pair_lane<N, U, B>(Ms, C - 1, (uint64_t)wq * 64 + lane, (uint64_t)wpm * 64, iters, are, aim);
+  are = wave_sum(are);
+  aim = wave_sum(aim);
+  if (lane == 0) {
+    const cd k = Km;
+    partial[(size_t)b * wpm + wq] = {ldexp(k.re * are - k.im * aim, Ke),
+                                     ldexp(k.re * aim + k.im * are, Ke)};
+  }
 }
 
 // NARROW: one lane per matrix, whole enumeration in the lane (small n), global rows.
@@ -354,6 +573,39 @@ inline int glynn_u_max(int n) {
 }
 
 constexpr int kWideMinN = 11;  // below this a lane owns a whole matrix
+
+// FP64 sizes that run the paired walk first (the plain WIDE kernel then takes the
+// matrices it left): from where a chunk holds the static block plus one bit up to the
+// largest n that fits 256 VGPRs without spills (n = 40 spills).
+#ifndef TB_GLYNN_PAIR_MIN_N
+#define TB_GLYNN_PAIR_MIN_N 14
+#endif
+#ifndef TB_GLYNN_PAIR_MAX_N
+#define TB_GLYNN_PAIR_MAX_N 39
+#endif
+template <int N, typename F>
+constexpr bool glynn_paired() {
+  return sizeof(F) == 8 && N >= TB_GLYNN_PAIR_MIN_N && N <= TB_GLYNN_PAIR_MAX_N;
+}
+// Static super-step bits of the paired walk and its rows in broadcast form (rows 1..B).
+template <int N>
+constexpr int glynn_pair_u() {
+  return N >= 25 ? 3 : 2;
+}
+template <int N>
+constexpr int glynn_pair_b() {
+  constexpr int b = (N <= TB_GLYNN_BCAST_WIDE_MAX_N) ? glynn_pair_u<N>() + 2 : glynn_pair_u<N>();
+  return b < N - 2 ? b : N - 2;
+}
+// Lower bound on C of a launch of n: the static bits, and for the paired walk its static
+// super-step bits plus the pair bit.
+template <int N, typename F>
+constexpr int glynn_min_c() {
+  constexpr int u = glynn_u<N, F>();
+  if constexpr (glynn_paired<N, F>())
+    return u > glynn_pair_u<N>() + 1 ? u : glynn_pair_u<N>() + 1;
+  return u;
+}
 constexpr int kMaxC = 12;      // Gray bits per lane chunk
 constexpr int kMaxWpmBits = 13;  // waves per matrix before lanes iterate over chunks
 
@@ -391,21 +643,34 @@ inline GlynnPlan glynn_plan(int n, int batch, int c_pref, int u = 2) {
 
 template <int N, typename F>
 inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, int batch, tb::cd *out,
-                              tb::cd *partial, int c_pref, hipStream_t s) {
+                              tb::cd *partial, int *handled, int c_pref, hipStream_t s) {
   if constexpr (N < kWideMinN) {
     constexpr int U = (N - 1) < 2 ? (N - 1) : 2;
     hipLaunchKernelGGL((tb::glynn_narrow_kernel<N, U, F>), dim3((batch + 255) / 256), dim3(256), 0,
                        s, A, mstride, batch, out);
     (void)partial;
+    (void)handled;
     (void)c_pref;
     return tbh::check_hip(hipGetLastError(), "glynn_narrow_kernel");
   } else {
     constexpr int U = glynn_u<N, F>();
-    static_assert(U <= N - 9, "the static block must fit the smallest chunk (C <= n - 9)");
-    const GlynnPlan p = glynn_plan(N, batch, c_pref, U);
+    constexpr int CMIN = glynn_min_c<N, F>();
+    static_assert(CMIN <= N - 9, "the static block must fit the smallest chunk (C <= n - 9)");
+    const GlynnPlan p = glynn_plan(N, batch, c_pref, CMIN);
     const unsigned blocks = (unsigned)batch * (unsigned)(p.wpm / 4);
+    const int *skip = nullptr;
+    if constexpr (glynn_paired<N, F>()) {
+      hipLaunchKernelGGL((tb::glynn_pair_kernel<N, glynn_pair_u<N>(), glynn_pair_b<N>()>),
+                         dim3(blocks), dim3(256), 0, s, (const tb::cpx<double> *)A, mstride, p.C,
+                         p.wpm, p.iters, partial, handled);
+      int32_t rc = tbh::check_hip(hipGetLastError(), "glynn_pair_kernel");
+      if (rc) return rc;
+      skip = handled;
+    } else {
+      (void)handled;
+    }
     hipLaunchKernelGGL((tb::glynn_wide_kernel<N, U, glynn_reg_rows<N, F>(), F>), dim3(blocks), dim3(256), 0, s,
-                       A, mstride, p.C, p.wpm, p.iters, partial);
+                       A, mstride, p.C, p.wpm, p.iters, partial, skip);
     int32_t rc = tbh::check_hip(hipGetLastError(), "glynn_wide_kernel");
     if (rc) return rc;
     hipLaunchKernelGGL(tb::glynn_finish_kernel, dim3((batch + 3) / 4), dim3(256), 0, s, partial,
@@ -416,17 +681,18 @@ inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, int batch, t
 
 template <typename F, int N = 2>
 inline int32_t dispatch_glynn(int n, const tb::cpx<F> *A, int64_t mstride, int batch, tb::cd *out,
-                              tb::cd *partial, int c_pref, hipStream_t s) {
+                              tb::cd *partial, int *handled, int c_pref, hipStream_t s) {
   if constexpr (N > TB_GLYNN_MAX_N) {
     return tbh::fail(TB_ERANGE, "glynn: n=%d exceeds TB_GLYNN_MAX_N=%d", n, TB_GLYNN_MAX_N);
   } else {
-    if (n == N) return launch_glynn_n<N, F>(A, mstride, batch, out, partial, c_pref, s);
-    return dispatch_glynn<F, N + 1>(n, A, mstride, batch, out, partial, c_pref, s);
+    if (n == N) return launch_glynn_n<N, F>(A, mstride, batch, out, partial, handled, c_pref, s);
+    return dispatch_glynn<F, N + 1>(n, A, mstride, batch, out, partial, handled, c_pref, s);
   }
 }
 
+// `handled`: batch ints of workspace for the paired FP64 walk (unused by FP32).
 int32_t glynn_dispatch_f64(int n, const tb::cpx<double> *A, int64_t mstride, int batch,
-                           tb::cd *out, tb::cd *partial, int c_pref, hipStream_t s);
+                           tb::cd *out, tb::cd *partial, int *handled, int c_pref, hipStream_t s);
 int32_t glynn_dispatch_f32(int n, const tb::cpx<float> *A, int64_t mstride, int batch,
                            tb::cd *out, tb::cd *partial, int c_pref, hipStream_t s);
 

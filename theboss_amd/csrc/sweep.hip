@@ -46,6 +46,111 @@ __global__ __launch_bounds__(64) TB_SWEEP_ATTR(K) void sweep_kernel(
   }
 }
 
+// Paired variant of sweep_kernel (same grid, tasks and partial layout). Per task the wave
+// stages the item, picks the particle whose smallest column entry is largest (relative to
+// the item's largest entry, at least 2^-16), swaps it into particle 1, divides the other
+// columns by it and runs the paired walk; an item without such a particle takes the plain
+// walk. The swap is undone when the minors are written. LDS: Vs | Ed | inv[R] | pmin[K] |
+// pmax[K] | Kc, exponent, pick.
+template <int K, int U>
+__global__ __launch_bounds__(64) TB_SWEEP_ATTR(K) void sweep_pair_kernel(
+    const cd *__restrict__ Vt, int ldv, int64_t item_stride, const int *__restrict__ item_map,
+    const int *__restrict__ item_count, int items, int c, int wpi, int iters,
+    cd *__restrict__ partial) {
+  constexpr int R = K - 1;
+  extern __shared__ cd smem[];
+  cd *Vs = smem;                                       // K*R
+  cd *Ed = smem + K * R;                               // (c - U) * 64
+  cd *inv = Ed + (size_t)(c > U ? c - U : 0) * 64;     // R
+  double *pmin = (double *)(inv + R);                  // K
+  double *pmax = pmin + K;                             // K
+  cd *kc = (cd *)(pmax + K);                           // 1 (2K doubles keep 16-byte alignment)
+  int *ke = (int *)(kc + 1);                           // exponent, pick
+  const int lane = threadIdx.x;
+  int nitems = items;
+  if (item_count) nitems = min(items, *item_count);
+  constexpr int kXcd = 8;
+  const int G = gridDim.x, p = blockIdx.x;
+  const int x = p % kXcd, bi = p / kXcd;
+  const int nblk_x = G / kXcd + (x < G % kXcd ? 1 : 0);
+  const int64_t total = (int64_t)nitems * wpi;
+  const int64_t per_x = (total + kXcd - 1) / kXcd;
+  const int64_t lo = (int64_t)x * per_x;
+  const int64_t hi = min(total, lo + per_x);
+  for (int64_t task = lo + bi; task < hi; task += nblk_x) {
+    const int slot = (int)(task / wpi);
+    const int w = (int)(task - (int64_t)slot * wpi);
+    const int item = item_map ? item_map[slot] : slot;
+    const cd *src = Vt + (size_t)item * item_stride;
+    __syncthreads();  // previous task's reads of LDS are done
+    for (int e = lane; e < K * R; e += 64) {
+      const int nu = e / R, j = e - nu * R;
+      Vs[e] = src[(size_t)nu * ldv + j];
+    }
+    __syncthreads();
+    if (lane < K) {
+      double mn = INFINITY, mx = 0.0;
+      for (int j = 0; j < R; ++j) {
+        const cd v = Vs[lane * R + j];
+        const double a = fma(v.re, v.re, v.im * v.im);
+        mn = fmin(mn, a);
+        mx = fmax(mx, a);
+      }
+      pmin[lane] = mn;
+      pmax[lane] = mx;
+    }
+    __syncthreads();
+    if (lane == 0) {
+      int best = 0;
+      double mx = 0.0;
+      for (int nu = 0; nu < K; ++nu) {
+        if (pmin[nu] > pmin[best]) best = nu;
+        mx = fmax(mx, pmax[nu]);
+      }
+      const bool ok = mx > 0.0 && mx <= 1.7976931348623157e308 && pmin[best] >= kPairMinRel2 * mx;
+      ke[1] = ok ? best : -1;
+    }
+    __syncthreads();
+    const int pick = ke[1];
+    cd acc;
+    int dst = lane;  // minor slot this lane's result belongs to
+    if (pick < 0) {
+      acc = sweep_wave<K, U, 3>(Vs, Ed, lane, c, w, wpi, iters);
+    } else {
+      if (pick != 1 && lane < R) {
+        const cd t = Vs[R + lane];
+        Vs[R + lane] = Vs[pick * R + lane];
+        Vs[pick * R + lane] = t;
+      }
+      __syncthreads();
+      if (lane < R) inv[lane] = cinv_scaled(Vs[R + lane]);
+      if (lane == 63) {
+        cd m;
+        int e2;
+        prod_half_scaled(Vs + R, R, m, e2);
+        kc[0] = m;
+        ke[0] = e2;
+      }
+      __syncthreads();
+      for (int e = lane; e < K * R; e += 64) {
+        const int nu = e / R, j = e - nu * R;
+        if (nu != 1) {
+          const cd v = cmul(Vs[e], inv[j]);
+          Vs[e] = {2.0 * v.re, 2.0 * v.im};
+        }
+      }
+      __syncthreads();
+      const cd a = sweep_pair_wave<K, U>(Vs, Ed, lane, c, w, wpi, iters);
+      const cd m = kc[0];
+      const int e2 = ke[0];
+      acc = {ldexp(m.re * a.re - m.im * a.im, e2), ldexp(m.re * a.im + m.im * a.re, e2)};
+      if (lane == 1) dst = pick;
+      else if (lane == pick) dst = 1;
+    }
+    if (lane < K) partial[((size_t)slot * wpi + w) * K + dst] = acc;
+  }
+}
+
 // minors[slot*K + nu] = sum_w partial[(slot*wpi + w)*K + nu], fixed order.
 __global__ void sweep_reduce_kernel(const cd *__restrict__ partial, int K, int wpi, int items,
                                     const int *__restrict__ item_count, cd *__restrict__ minors) {
@@ -116,10 +221,25 @@ inline SweepPlan sweep_plan(int k, int64_t items, int c_override) {
   return p;
 }
 
-// LDS per wave: the item's k(k-1) columns plus 64 lanes x (c - U) corrector slots.
+// LDS per wave: the item's k(k-1) columns plus 64 lanes x (c - U) corrector slots; the
+// paired kernel adds the quotients' reciprocals, the particles' extreme magnitudes and the
+// column product.
 inline size_t sweep_lds_bytes(int k, int c) {
   const int u = std::min(tb::kSweepU, k - 1);
   return sizeof(tb::cd) * ((size_t)k * (k - 1) + (size_t)std::max(c - u, 0) * 64);
+}
+inline size_t sweep_pair_lds_bytes(int k, int c) {
+  return sweep_lds_bytes(k, c) + sizeof(tb::cd) * (size_t)(k - 1) +
+         sizeof(double) * (size_t)(2 * k) + sizeof(tb::cd) + 2 * sizeof(int);
+}
+// Sizes that run the paired sweep (with a chunk holding the static block plus the pair
+// bit): up to the largest k that keeps 2 waves per SIMD without spills.
+#ifndef TB_SWEEP_PAIR_MAX_K
+#define TB_SWEEP_PAIR_MAX_K 21
+#endif
+template <int K>
+constexpr bool sweep_paired() {
+  return K >= tb::kSweepPairMinK && K <= TB_SWEEP_PAIR_MAX_K;
 }
 
 template <int K>
@@ -131,6 +251,14 @@ int32_t launch_sweep_k(const tb::cd *Vt, int ldv, int64_t item_stride, const int
   const unsigned grid = (unsigned)std::min<int64_t>(tasks, tbs::kSweepMaxBlocks);
   constexpr int R0 = (K <= TB_SWEEP_BCAST_MAX_K) ? 3 : 2;
   constexpr int U = (K - 1 < tb::kSweepU) ? K - 1 : tb::kSweepU;
+  if constexpr (sweep_paired<K>()) {
+    if (p.c >= U + 1) {
+      hipLaunchKernelGGL((tb::sweep_pair_kernel<K, U>), dim3(grid), dim3(64),
+                         sweep_pair_lds_bytes(K, p.c), s, Vt, ldv, item_stride, item_map,
+                         item_count, items, p.c, p.wpi, p.iters, partial);
+      return tbh::check_hip(hipGetLastError(), "sweep_pair_kernel");
+    }
+  }
   hipLaunchKernelGGL((tb::sweep_kernel<K, U, R0>), dim3(grid), dim3(64), sweep_lds_bytes(K, p.c), s,
                      Vt, ldv, item_stride, item_map, item_count, items, p.c, p.wpi, p.iters,
                      partial);

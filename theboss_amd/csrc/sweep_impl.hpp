@@ -329,4 +329,187 @@ __device__ __forceinline__ cd sweep_wave(const cd *Vs, cd *Ed, int lane, int c, 
   return acc;
 }
 
+// ------------------------------------------------------------------------------------
+// Paired sweep (kSweepPairMinK <= k <= TB_SWEEP_PAIR_MAX_K). Gray bit 0 is particle 1;
+// codes 2t and 2t+1 differ only in it. With the columns of every particle nu != 1 divided
+// by particle 1's (S2[nu][j] = 2 V[nu][j] / V[1][j], u_j = (1/2) sum_{nu != 1} delta_nu
+// S2[nu][j], delta_0 = +1) the two codes' half sums are (V[1][j]/2)(u_j +- 1), so with
+// X = sum_t (-1)^t prod(u + 1), Y = sum_t (-1)^t prod(u - 1) over the Gray ranks t of
+// particles 2..k-1 and Kc = prod_j V[1][j] / 2:
+//   minor_0 = Kc (X - Y), minor_1 = Kc (X + Y),
+//   minor_{b+2} = Kc (E_b + delta_{b+2}(end) (X - Y))
+// with the per-bit correctors E_b of the plain sweep kept on T = X - Y. Per pair of codes:
+// 2R flip instructions, 2 (4R - 2) for the products and 4 for the corrector, 10R in all
+// against 2 (6R - 2) for two plain steps (R = k - 1).
+// ------------------------------------------------------------------------------------
+constexpr int kSweepPairMinK = 8;
+
+// Particles 2..NU+1 in broadcast form.
+template <int R>
+struct ColP {
+  static constexpr int NU = (kSweepU + 2 < R - 1) ? kSweepU + 2 : R - 1;
+  Bcast<R> b[NU];
+  double one;
+};
+
+template <int R, int MODE>
+__device__ __forceinline__ void sweep_pair_flip(double (&ur)[R], double (&ui)[R], const cd *S2,
+                                                const ColP<R> &cp, int b, double sg) {
+  if (b < ColP<R>::NU) {  // compile-time at the static flip sites, uniform at block starts
+#pragma unroll
+    for (int u = 0; u < ColP<R>::NU; ++u)
+      if (u == b) flip_bcast<0, R, MODE>(ur, ui, cp.b[u], MODE == 0 ? sg : cp.one);
+    return;
+  }
+  sweep_flip<R, MODE>(ur, ui, S2 + (b + 2) * R, sg);
+}
+
+// One chunk of 2^c super-steps (Gray bits of particles 2..k-1) starting at t0 = q << c.
+// mi[nu] receives the chunk's sums without the factor Kc.
+template <int K, int U>
+__device__ __forceinline__ void sweep_pair_chunk(const cd *S2, const ColP<K - 1> &cp, cd *Ed, int c,
+                                                 uint64_t q, cd (&mi)[K]) {
+  constexpr int R = K - 1;
+  constexpr int NB = K - 2;  // Gray bits: bit b <-> particle b + 2
+  const uint64_t t0 = q << c;
+  const uint64_t g0 = t0 ^ (t0 >> 1);
+  double ur[R], ui[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    ur[j] = 0.5 * S2[j].re;
+    ui[j] = 0.5 * S2[j].im;
+  }
+#pragma unroll 1
+  for (int b = 0; b < NB; ++b) {
+    const double d = ((g0 >> b) & 1ull) ? -0.5 : 0.5;
+    const cd *col = S2 + (b + 2) * R;
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      ur[j] = fma(d, col[j].re, ur[j]);
+      ui[j] = fma(d, col[j].im, ui[j]);
+    }
+  }
+  double Xr = 0.0, Xi = 0.0, Yr = 0.0, Yi = 0.0;
+  if (t0 & 1ull)
+    pair_acc2<-1>(Xr, Xi, Yr, Yi, ur, ui);
+  else
+    pair_acc2<1>(Xr, Xi, Yr, Yi, ur, ui);
+  double Esr[U], Esi[U];
+#pragma unroll
+  for (int i = 0; i < U; ++i) {
+    Esr[i] = 0.0;
+    Esi[i] = 0.0;
+  }
+  for (int i = U; i < c; ++i) Ed[(i - U) * 64] = {0.0, 0.0};
+  const bool q_odd = (q & 1ull) != 0;
+  const int nblocks = 1 << (c - U);
+  // Flip of bit i at local step l: t_{i+1} == 0: delta goes +1 -> -1 (u -= S2, corrector
+  // += 2T); t_{i+1} == 1: delta goes -1 -> +1 (u += S2, corrector -= 2T); T = X - Y.
+  auto static_steps = [&](int blk) {
+#pragma unroll
+    for (int j = 1; j < (1 << U); ++j) {
+      const int i = __builtin_ctz(j);
+      const double Tr = Xr - Yr, Ti = Xi - Yi;
+      if (i + 1 < U) {
+        if ((j >> (i + 1)) & 1) {
+          Esr[i] = fma(-2.0, Tr, Esr[i]);
+          Esi[i] = fma(-2.0, Ti, Esi[i]);
+          sweep_pair_flip<R, 1>(ur, ui, S2, cp, i, 1.0);
+        } else {
+          Esr[i] = fma(2.0, Tr, Esr[i]);
+          Esi[i] = fma(2.0, Ti, Esi[i]);
+          sweep_pair_flip<R, -1>(ur, ui, S2, cp, i, -1.0);
+        }
+      } else {
+        const bool t_next = (U < c) ? ((blk & 1) != 0) : q_odd;
+        const double sg = t_next ? 1.0 : -1.0;
+        Esr[i] = fma(-2.0 * sg, Tr, Esr[i]);
+        Esi[i] = fma(-2.0 * sg, Ti, Esi[i]);
+        sweep_pair_flip<R, 0>(ur, ui, S2, cp, i, sg);
+      }
+      if (j & 1)
+        pair_acc2<-1>(Xr, Xi, Yr, Yi, ur, ui);
+      else
+        pair_acc2<1>(Xr, Xi, Yr, Yi, ur, ui);
+      TB_STEP_FENCE();
+    }
+  };
+  static_steps(0);
+  for (int blk = 1; blk < nblocks; ++blk) {
+    const int i = U + __builtin_ctz(blk);
+    const bool t_next = (i + 1 < c) ? (((blk >> (i + 1 - U)) & 1) != 0) : q_odd;
+    const double sg = t_next ? 1.0 : -1.0;
+    cd e = Ed[(i - U) * 64];
+    e.re = fma(-2.0 * sg, Xr - Yr, e.re);
+    e.im = fma(-2.0 * sg, Xi - Yi, e.im);
+    Ed[(i - U) * 64] = e;
+    if (i < ColP<R>::NU)
+      sweep_pair_flip<R, 0>(ur, ui, S2, cp, i, sg);
+    else
+      sweep_flip<R, 0>(ur, ui, S2 + (i + 2) * R, sg);
+    pair_acc2<1>(Xr, Xi, Yr, Yi, ur, ui);
+    TB_STEP_FENCE();
+    static_steps(blk);
+  }
+  const double Tr = Xr - Yr, Ti = Xi - Yi;
+  const uint64_t tend = t0 + ((uint64_t)1 << c) - 1;
+  const uint64_t gend = tend ^ (tend >> 1);
+  mi[0] = {Tr, Ti};
+  mi[1] = {Xr + Yr, Xi + Yi};
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const double d = ((gend >> b) & 1ull) ? -1.0 : 1.0;
+    if (b < c) {
+      double er, ei;
+      if (b < U) {
+        er = Esr[b < U ? b : 0];
+        ei = Esi[b < U ? b : 0];
+      } else {
+        const cd e = Ed[(b - U) * 64];
+        er = e.re;
+        ei = e.im;
+      }
+      mi[b + 2] = {fma(d, Tr, er), fma(d, Ti, ei)};
+    } else {
+      mi[b + 2] = {d * Tr, d * Ti};
+    }
+  }
+}
+
+// One wave's share of an item whose scaled columns are in S2, chunk c of codes (c - 1
+// super-step bits per lane): the chunk and lane mapping of sweep_wave. Lane i returns
+// minor i's sum without Kc.
+template <int K, int U>
+__device__ __forceinline__ cd sweep_pair_wave(const cd *S2, cd *Ed, int lane, int c, int w,
+                                              int wpi, int iters) {
+  constexpr int R = K - 1;
+  const uint64_t nchunks = (uint64_t)1 << (K - 1 - c);
+  ColP<R> cp;
+#pragma unroll
+  for (int u = 0; u < ColP<R>::NU; ++u) load_bcast(cp.b[u], (const double *)(S2 + (u + 2) * R), lane);
+  cp.one = vgpr_one();
+  asm volatile("s_nop 4");  // VALU writes of the store before its first DPP read
+  cd acc = {0.0, 0.0};
+  for (int it = 0; it < iters; ++it) {
+    const uint64_t q = ((uint64_t)w + (uint64_t)it * wpi) * 64 + lane;
+    cd mi[K];
+    const bool live = q < nchunks;
+    sweep_pair_chunk<K, U>(S2, cp, Ed + lane, c - 1, live ? q : (q & (nchunks - 1)), mi);
+    if (!live) {
+#pragma unroll
+      for (int i = 0; i < K; ++i) mi[i] = {0.0, 0.0};
+    }
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const double re = wave_sum(mi[i].re);
+      const double im = wave_sum(mi[i].im);
+      if (lane == i) {
+        acc.re += re;
+        acc.im += im;
+      }
+    }
+  }
+  return acc;
+}
+
 }  // namespace tb
