@@ -238,12 +238,14 @@ def main() -> int:
         "fp64_flops_per_sample": sample_flops(N, M),
         "fp64_tflops_sampler": batched * sample_flops(N, M) / 1e12,
         "roofline": {
-            "kernel": "sweep_kernel<20,3,3> (C&C minors sweep, k=20, 4096 items, chunk 2^%d)"
+            "kernel": "sweep_pair_kernel<20,3> (C&C minors sweep, paired walk, k=20, 4096 items, chunk 2^%d)"
                       % (tuned["chunk_bits"].get(N, autotune.sweep_builtin(N, SAMPLES)) if tuned
                          else autotune.sweep_builtin(N, SAMPLES)),
             "flops_per_launch": SAMPLES * sweep_flops(k),
-            "flop_count": "2^(k-1) (10k - 12) per item: the Glynn-type sweep's executed terms "
-                          "(a Ryser walk's 2^k (10k - 12) would double the numerator)",
+            "flop_count": "2^(k-1) (10k - 12) per item: the Glynn-type sweep's terms, as the FLOP "
+                          "counter reads them (FMAs as 2 flops; the paired walk executes 10k - 13 "
+                          "per code by that count, the plain walk 10k - 10); a Ryser walk's "
+                          "2^k (10k - 12) would double the numerator",
             "clock_ghz": sweep_ghz,
             "bound": "valu",
             "achieved": achieved,
