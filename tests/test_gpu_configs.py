@@ -172,9 +172,10 @@ def test_fp32_colsum_rule_boundary_n23_n24(T, oracle):
 # FP32 column sums below the rule's limit: max relative error of these 8 Haar blocks
 # against the oracle, bounded at about 3x the value measured on MI355X. The error varies
 # from block to block with the cancellation of the Glynn sum: over 64 blocks (seeds
-# 5000 + n + b) the median / 90th percentile / max measured 9e-7 / 1.8e-6 / 4.7e-6 (n = 8),
-# 3e-6 / 1.3e-5 / 3.6e-5 (n = 16), 8e-6 / 2.3e-5 / 4.5e-5 (n = 20) and 4e-5 / 1.1e-4 / 4.8e-4
-# (n = 23).
+# 5000 + n + b) the median / 90th percentile / max measured 9e-7 / 1.8e-6 / 4.7e-6 (n = 8,
+# one lane per matrix), 7.8e-7 / 2.3e-6 / 6.5e-6 (n = 12, plain walk), and with the paired
+# walk 1.7e-6 / 5.9e-6 / 2.9e-5 (n = 16), 4.0e-6 / 1.1e-5 / 2.1e-5 (n = 20) and
+# 9.3e-6 / 2.6e-5 / 8.1e-5 (n = 23).
 FP32_SMALL_BOUND = {8: 6e-6, 12: 6e-6, 16: 5e-5, 20: 7e-5, 23: 9e-5}
 
 

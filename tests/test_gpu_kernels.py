@@ -121,6 +121,46 @@ def test_glynn_paired_walk_and_plain_fallback_in_one_batch(T, ops, oracle, n):
     assert abs(ref[-1] - ref[0] * 2.0 ** (-30 * n)) <= 1e-12 * abs(ref[-1])
 
 
+@pytest.mark.parametrize("n", [14, 18, 23])
+def test_glynn_fp32_paired_walk_and_plain_fallback_in_one_batch(T, ops, oracle, n):
+    """FP32 column sums at 14 <= n <= 23 run the paired walk too (quotients by the pair row
+    formed in FP64 from the FP64 matrix, then FP32 sums and products) and leave matrices
+    without a row clear of small entries to the plain FP32 walk. The same mix of blocks as
+    the FP64 test, minus the 2^-30 scaling (the plain FP32 walk keeps no exponent), each
+    within 3e-4 relative of the oracle (the FP32 error table's largest measured error at
+    these n is 4.8e-4 at n = 23, over 64 Haar blocks; these blocks sit far below it)."""
+    from theboss_amd import _lib
+
+    rng = np.random.default_rng(400 + n)
+    H = [oracle.haar_unitary(2 * n, 4000 + 10 * n + k)[:n, :n] for k in range(6)]
+    mats = [H[0]]
+    Z = H[1].copy()
+    Z[np.arange(n), rng.integers(0, n, n)] = 0.0
+    mats.append(Z)
+    for k, dest in ((2, n - 1), (3, n // 2)):
+        A = H[k].copy()
+        best = int(np.argmax(np.min(np.abs(A), axis=1)))
+        A[[best, dest]] = A[[dest, best]]
+        mats.append(A)
+    A = H[4].copy()
+    keep = int(rng.integers(0, n))
+    for i in range(n):
+        if i != keep:
+            A[i, rng.integers(0, n)] = 2.0 ** -20 * np.max(np.abs(A))
+    mats.append(A)
+    A = H[5].copy()
+    A[np.arange(n), rng.integers(0, n, n)] = 2.0 ** -17 * np.max(np.abs(A))
+    mats.append(A)
+    As = np.stack(mats)
+    got = ops.glynn_batch(dev_c(T, As), _lib.TB_PREC_FP32_COLSUM).cpu().numpy()
+    f64 = ops.glynn_batch(dev_c(T, As), _lib.TB_PREC_FP64).cpu().numpy()
+    ref = oracle.perm_glynn_batch(As, 1)
+    err = np.abs(got - ref) / np.abs(ref)
+    print(f"fp32 paired/plain n={n}: rel err per block {np.array2string(err, precision=2)}")
+    assert err.max() <= 3e-4, (n, err)
+    assert not np.array_equal(got, f64)  # FP32 kernels ran, not the FP64 ones
+
+
 @pytest.mark.parametrize("n", [39, 40, 41])
 def test_glynn_fp64_rank_one_large(T, ops, oracle, n):
     """Near the largest supported size (2^38..2^40 Gray steps; n = 39 is the largest with

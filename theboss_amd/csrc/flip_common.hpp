@@ -49,9 +49,8 @@ struct TreeProdT<F, LO, LO + 1> {
 // MODE 1: h += row, MODE -1: h -= row, MODE 0: h += sg * row (runtime direction, one
 // FMA per component with sg = +-1 held per lane).
 // Columns J0..N-1 only (J0 > 0 when the first J0 values of the row are held in SGPRs).
-template <int N, int MODE, typename F, typename P, int J0 = 0>
+template <int N, int MODE, typename F, typename P, int J0 = 0, int G = TB_FLIP_GROUP>
 __device__ __forceinline__ void flip_mem(F (&hr)[N], F (&hi)[N], P row, F sg) {
-  constexpr int G = TB_FLIP_GROUP;
   constexpr int NG = (N - J0 + G - 1) / G;
   cpx<F> cur[G], nxt[G];
 #pragma unroll

@@ -3,8 +3,10 @@
 #include "glynn_impl.hpp"
 
 namespace tbgl {
-int32_t glynn_dispatch_f32(int n, const tb::cpx<float> *A, int64_t mstride, int batch,
-                           tb::cd *out, tb::cd *partial, int c_pref, hipStream_t s) {
-  return dispatch_glynn<float>(n, A, mstride, batch, out, partial, nullptr, c_pref, s);
+int32_t glynn_dispatch_f32(int n, const tb::cpx<float> *A, int64_t mstride, const tb::cd *A64,
+                           int64_t mstride64, int batch, tb::cd *out, tb::cd *partial,
+                           int *handled, int c_pref, hipStream_t s) {
+  return dispatch_glynn<float>(n, A, mstride, A64, mstride64, batch, out, partial, handled,
+                               c_pref, s);
 }
 }  // namespace tbgl

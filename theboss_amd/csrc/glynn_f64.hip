@@ -7,7 +7,8 @@
 namespace tbgl {
 int32_t glynn_dispatch_f64(int n, const tb::cpx<double> *A, int64_t mstride, int batch,
                            tb::cd *out, tb::cd *partial, int *handled, int c_pref, hipStream_t s) {
-  return dispatch_glynn<double>(n, A, mstride, batch, out, partial, handled, c_pref, s);
+  return dispatch_glynn<double>(n, A, mstride, (const tb::cd *)A, mstride, batch, out, partial,
+                                handled, c_pref, s);
 }
 }  // namespace tbgl
 
@@ -19,10 +20,12 @@ size_t tbgl_flag_bytes(int batch) { return align256((size_t)batch * sizeof(int))
 
 size_t tbgl_partial_bytes(int n, int batch, int c_pref) {
   if (n < tbgl::kWideMinN) return 0;
-  // large enough for the plan of either precision (their lower bounds on C may differ;
-  // a larger bound only shrinks wpm)
-  const int wpm = std::max(tbgl::glynn_plan(n, batch, c_pref, 2).wpm,
-                           tbgl::glynn_plan(n, batch, c_pref, tbgl::glynn_u_max(n)).wpm);
+  // large enough for the plan of either precision and either residency (their lower
+  // bounds on C may differ, and a larger bound only shrinks wpm)
+  int wpm = 0;
+  for (int u : {2, tbgl::glynn_u_max(n)})
+    for (int slots : {2, TB_GLYNN_F32_PAIR_WPE})
+      wpm = std::max(wpm, tbgl::glynn_plan(n, batch, c_pref, u, slots).wpm);
   return tbgl_flag_bytes(batch) + align256((size_t)batch * wpm * sizeof(tb::cd));
 }
 
@@ -90,8 +93,9 @@ extern "C" int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t bat
                      (const tb::cd *)A, matrix_stride, n, batch, A32);
   int32_t rc = tbh::check_hip(hipGetLastError(), "to_fp32_kernel");
   if (rc) return rc;
-  return tbgl::glynn_dispatch_f32(n, (const tb::cpx<float> *)A32, (int64_t)n * n, batch, o,
-                                    partial, c_pref, s);
+  return tbgl::glynn_dispatch_f32(n, (const tb::cpx<float> *)A32, (int64_t)n * n,
+                                  (const tb::cd *)A, matrix_stride, batch, o, partial, handled,
+                                  c_pref, s);
 }
 
 // ------------------------------------------------------------------------------------

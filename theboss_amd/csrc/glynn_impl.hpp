@@ -465,6 +465,315 @@ __global__ __launch_bounds__(256) TB_WIDE_ATTR(N) void glynn_pair_kernel(
   }
 }
 
+// ------------------------------------------------------------------------------------
+// Paired walk with FP32 column sums and products (14 <= n <= TB_FP32_COLSUM_MAX_N): the
+// walk of glynn_pair_lane in float, the quotients 2 M[i][j] / M[0][j] formed in FP64 and
+// rounded once; static row 1 in SGPRs (SR = 1), rows SR + 1..U in plain VGPRs. Each pair's
+// difference prod(u + 1) - prod(u - 1) is formed in FP32, summed in FP32 over the 2^U
+// pairs of a block and accumulated in FP64 once per block.
+// ------------------------------------------------------------------------------------
+// Complex values per LDS read group of the paired FP32 kernel's memory flips (two
+// groups in flight).
+#ifndef TB_GLYNN_F32_PAIR_FLIP_G
+#define TB_GLYNN_F32_PAIR_FLIP_G 4
+#endif
+// One step of the chain x <- x (u + PM), split in its two FMA levels.
+template <int PM>
+__device__ __forceinline__ void chain_lvl1(float xr, float xi, float u_r, float u_i, float &tr,
+                                           float &ti) {
+  tr = fmaf(xr, u_r, (float)PM * xr);
+  ti = fmaf(xr, u_i, (float)PM * xi);
+}
+__device__ __forceinline__ void chain_lvl2(float &xr, float &xi, float u_r, float u_i, float tr,
+                                           float ti) {
+  const float nr = fmaf(-xi, u_i, tr);
+  xi = fmaf(xi, u_r, ti);
+  xr = nr;
+}
+
+// SIGN * (Π(u + 1) - Π(u - 1)) added to the FP32 block sums (ar, ai). Four chains run side by side, Π(u ± 1)
+// over columns [0, H) and [H, N): a chain step is two dependent FMA levels, and FP32
+// issues fast enough that one chain alone waits on its own results; with the four
+// interleaved level by level, each FMA's operand was produced eight instructions earlier.
+template <int SIGN, int N>
+__device__ __forceinline__ void pair_acc_f32(float &ar, float &ai, const float (&ur)[N],
+                                             const float (&ui)[N]) {
+  constexpr int H = N / 2;  // the upper half is the longer one for odd N
+  float pr = ur[0] + 1.f, pi = ui[0];  // (u + 1), columns [0, H)
+  float qr = ur[H] + 1.f, qi = ui[H];  // (u + 1), columns [H, N)
+  float mr = ur[0] - 1.f, mi = ui[0];  // (u - 1), columns [0, H)
+  float wr = ur[H] - 1.f, wi = ui[H];  // (u - 1), columns [H, N)
+#pragma unroll
+  for (int k = 1; k < N - H; ++k) {
+    float t[8];
+    if (k < H) {
+      chain_lvl1<1>(pr, pi, ur[k], ui[k], t[0], t[1]);
+      chain_lvl1<-1>(mr, mi, ur[k], ui[k], t[4], t[5]);
+    }
+    chain_lvl1<1>(qr, qi, ur[H + k], ui[H + k], t[2], t[3]);
+    chain_lvl1<-1>(wr, wi, ur[H + k], ui[H + k], t[6], t[7]);
+    if (k < H) {
+      chain_lvl2(pr, pi, ur[k], ui[k], t[0], t[1]);
+      chain_lvl2(mr, mi, ur[k], ui[k], t[4], t[5]);
+    }
+    chain_lvl2(qr, qi, ur[H + k], ui[H + k], t[2], t[3]);
+    chain_lvl2(wr, wi, ur[H + k], ui[H + k], t[6], t[7]);
+  }
+  // (p q) - (m w) with the difference folded into the FMAs
+  const float mwr = fmaf(-mi, wi, mr * wr), mwi = fmaf(mi, wr, mr * wi);
+  const float dr = fmaf(pr, qr, -fmaf(pi, qi, mwr));
+  const float di = fmaf(pr, qi, fmaf(pi, qr, -mwi));
+  if constexpr (SIGN > 0) {
+    ar += dr;
+    ai += di;
+  } else {
+    ar -= dr;
+    ai -= di;
+  }
+}
+
+// h += v, h -= v or h += sg v for the static row whose first S values sit in SGPRs (s)
+// and the rest in VGPRs (rr, ri; entries below S unused).
+template <int N, int MODE, int S>
+__device__ __forceinline__ void flip_split_f32(float (&hr)[N], float (&hi)[N],
+                                               const SRow<S, float> &s, const float (&rr)[N],
+                                               const float (&ri)[N], float sg) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const float vr = j < S ? s.re[j < S ? j : 0] : rr[j];
+    const float vi = j < S ? s.im[j < S ? j : 0] : ri[j];
+    if constexpr (MODE == 1) {
+      hr[j] += vr;
+      hi[j] += vi;
+    } else if constexpr (MODE == -1) {
+      hr[j] -= vr;
+      hi[j] -= vi;
+    } else {
+      hr[j] = fmaf(sg, vr, hr[j]);
+      hi[j] = fmaf(sg, vi, hi[j]);
+    }
+  }
+}
+
+template <int N, int U, int SR, int S2P>
+__device__ __forceinline__ void glynn_pair_lane_f32(const cpx<float> *S2, int C, uint64_t q0,
+                                                    uint64_t qstride, int iters, double &acc_re,
+                                                    double &acc_im) {
+  constexpr int NB = N - 2;  // Gray bits: row i + 1 at bit i
+  const bool topbit_fixed = (C == NB);
+  // Row 1 (half of all flips, always a fixed direction) in SGPRs when SR = 1, as the
+  // scalar operand of the adds; rows SR + 1..U in VGPRs, except the first S2P values of
+  // row 2 (SR = 1), which also sit in SGPRs.
+  static_assert(S2P == 0 || (SR == 1 && U >= 2), "row 2 splits only beside an SGPR row 1");
+  SRow<SR ? N : 0, float> sr;
+  if constexpr (SR) load_srow<N>(sr, S2 + N);
+  SRow<S2P, float> sr2;
+  if constexpr (S2P > 0) load_srow<S2P>(sr2, S2 + 2 * N);
+  RegRows<N, U - SR, float> rr;
+#pragma unroll
+  for (int r = 0; r < U - SR; ++r)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      if (r == 0 && j < S2P) continue;
+      const cpx<float> v = S2[(size_t)(r + SR + 1) * N + j];
+      rr.re[r][j] = v.re;
+      rr.im[r][j] = v.im;
+    }
+  for (int it = 0; it < iters; ++it) {
+    const uint64_t q = q0 + (uint64_t)it * qstride;
+    const uint64_t t0 = q << C;
+    const uint64_t g0 = t0 ^ (t0 >> 1);
+    float ur[N], ui[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const cpx<float> v = S2[(size_t)(N - 1) * N + j];
+      ur[j] = v.re;
+      ui[j] = v.im;
+    }
+#pragma unroll 1
+    for (int i = 0; i < NB; ++i) {
+      const float sg = ((g0 >> i) & 1ull) ? -1.0f : 1.0f;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const cpx<float> v = S2[(size_t)(i + 1) * N + j];
+        ur[j] = fmaf(sg, v.re, ur[j]);
+        ui[j] = fmaf(sg, v.im, ui[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      ur[j] *= 0.5f;
+      ui[j] *= 0.5f;
+    }
+    const bool q_odd = (q & 1ull) != 0;
+    // Each block's pair terms (the block-start pair and the 2^U - 1 static ones) are
+    // summed in FP32, then added to the FP64 lane sums: one conversion and one FP64 add
+    // per block and component.
+    float bre = 0.f, bim = 0.f;
+    pair_acc_f32<1>(bre, bim, ur, ui);
+    const int nblocks = 1 << (C - U);
+    // row i + 1: mode +1 / -1 (fixed direction) or 0 (direction sg); every call site
+    // passes constants, so the branches fold away after unrolling
+    auto do_flip = [&](int i, int mode, float sg) {
+      if (SR && i == 0) {
+        if constexpr (SR) {
+          if (mode == 1)
+            flip_reg<N, 1, float>(ur, ui, sr.re, sr.im, sg);
+          else if (mode == -1)
+            flip_reg<N, -1, float>(ur, ui, sr.re, sr.im, sg);
+          else
+            flip_reg<N, 0, float>(ur, ui, sr.re, sr.im, sg);
+        }
+        return;
+      }
+      if (S2P > 0 && i == 1) {
+        if constexpr (S2P > 0) {
+          if (mode == 1)
+            flip_split_f32<N, 1, S2P>(ur, ui, sr2, rr.re[0], rr.im[0], sg);
+          else if (mode == -1)
+            flip_split_f32<N, -1, S2P>(ur, ui, sr2, rr.re[0], rr.im[0], sg);
+          else
+            flip_split_f32<N, 0, S2P>(ur, ui, sr2, rr.re[0], rr.im[0], sg);
+        }
+        return;
+      }
+      if (i < U) {
+#pragma unroll
+        for (int r = 0; r < U - SR; ++r)
+          if (r + SR == i) {
+            if (mode == 1)
+              flip_reg<N, 1, float>(ur, ui, rr.re[r], rr.im[r], sg);
+            else if (mode == -1)
+              flip_reg<N, -1, float>(ur, ui, rr.re[r], rr.im[r], sg);
+            else
+              flip_reg<N, 0, float>(ur, ui, rr.re[r], rr.im[r], sg);
+          }
+        return;
+      }
+      flip_mem<N, 0, float, const cpx<float> *, 0, TB_GLYNN_F32_PAIR_FLIP_G>(ur, ui, S2 + (size_t)(i + 1) * N, sg);
+    };
+    auto static_steps = [&](int blk) {
+#pragma unroll
+      for (int j = 1; j < (1 << U); ++j) {
+        const int i = __builtin_ctz(j);
+        if (i + 1 < U) {
+          if ((j >> (i + 1)) & 1)
+            do_flip(i, 1, 1.0f);
+          else
+            do_flip(i, -1, -1.0f);
+        } else {
+          bool add;
+          if (U < C)
+            add = (blk & 1) != 0;
+          else
+            add = topbit_fixed ? false : q_odd;
+          do_flip(i, 0, add ? 1.0f : -1.0f);
+        }
+        if (j & 1)
+          pair_acc_f32<-1>(bre, bim, ur, ui);
+        else
+          pair_acc_f32<1>(bre, bim, ur, ui);
+        TB_STEP_FENCE();
+      }
+      acc_re += (double)bre;
+      acc_im += (double)bim;
+      bre = 0.f;
+      bim = 0.f;
+    };
+    static_steps(0);
+    for (int blk = 1; blk < nblocks; ++blk) {
+      const int i = U + __builtin_ctz(blk);
+      bool add;
+      if (i + 1 < C)
+        add = ((blk >> (i + 1 - U)) & 1) != 0;
+      else
+        add = topbit_fixed ? false : q_odd;
+      flip_mem<N, 0, float, const cpx<float> *, 0, TB_GLYNN_F32_PAIR_FLIP_G>(ur, ui, S2 + (size_t)(i + 1) * N, add ? 1.0f : -1.0f);
+      pair_acc_f32<1>(bre, bim, ur, ui);
+      TB_STEP_FENCE();
+      static_steps(blk);
+    }
+  }
+}
+
+// PAIRED WIDE kernel, FP32 column sums: reads the FP64 matrices (the quotients and the
+// row-0 product are formed in FP64), writes partials and handled[] like glynn_pair_kernel.
+template <int N, int U, int SR, int S2P, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void glynn_pair_kernel_f32(
+    const cd *__restrict__ A, int64_t mstride, int C, int wpm, int iters,
+    cd *__restrict__ partial, int *__restrict__ handled) {
+  __shared__ cd Ms[N * N];
+  __shared__ cpx<float> Sf[N * N];
+  __shared__ cd inv0[N];
+  __shared__ double rmin[N], rmax[N];
+  __shared__ int pick, Ke;
+  __shared__ cd Km;
+  const int lane = threadIdx.x & 63;
+  const int wgs_per_matrix = wpm >> 2;
+  const int b = blockIdx.x / wgs_per_matrix;
+  const int wg = blockIdx.x - b * wgs_per_matrix;
+  const int wq = wg * 4 + (threadIdx.x >> 6);
+  const cd *src = A + (size_t)b * mstride;
+  for (int e = threadIdx.x; e < N * N; e += 256) Ms[e] = src[e];
+  __syncthreads();
+  if (threadIdx.x < N) {
+    double lo = INFINITY, hi = 0.0;
+    for (int j = 0; j < N; ++j) {
+      const cd v = Ms[threadIdx.x * N + j];
+      const double a = fma(v.re, v.re, v.im * v.im);
+      lo = fmin(lo, a);
+      hi = fmax(hi, a);
+    }
+    rmin[threadIdx.x] = lo;
+    rmax[threadIdx.x] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int best = 0;
+    double mx = 0.0;
+    for (int i = 0; i < N; ++i) {
+      if (rmin[i] > rmin[best]) best = i;
+      mx = fmax(mx, rmax[i]);
+    }
+    const bool ok = mx > 0.0 && mx <= 1.7976931348623157e308 && rmin[best] >= kPairMinRel2 * mx;
+    pick = ok ? best : -1;
+    if (wg == 0) handled[b] = ok ? 1 : 0;
+  }
+  __syncthreads();
+  const int r = pick;
+  if (r < 0) return;  // uniform over the workgroup
+  if (r != 0 && threadIdx.x < N) {
+    const cd t = Ms[threadIdx.x];
+    Ms[threadIdx.x] = Ms[r * N + threadIdx.x];
+    Ms[r * N + threadIdx.x] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < N) inv0[threadIdx.x] = cinv_scaled(Ms[threadIdx.x]);
+  if (threadIdx.x == 64) {
+    cd k;
+    int ke;
+    prod_half_scaled(Ms, N, k, ke);
+    Km = k;
+    Ke = ke;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < (N - 1) * N; e += 256) {
+    const int i = 1 + e / N, j = e - (i - 1) * N;
+    const cd v = cmul(Ms[i * N + j], inv0[j]);
+    Sf[i * N + j] = {(float)(2.0 * v.re), (float)(2.0 * v.im)};
+  }
+  __syncthreads();
+  double are = 0.0, aim = 0.0;
+  glynn_pair_lane_f32<N, U, SR, S2P>(Sf, C - 1, (uint64_t)wq * 64 + lane, (uint64_t)wpm * 64, iters, are, aim);
+  are = wave_sum(are);
+  aim = wave_sum(aim);
+  if (lane == 0) {
+    const cd k = Km;
+    partial[(size_t)b * wpm + wq] = {ldexp(k.re * are - k.im * aim, Ke),
+                                     ldexp(k.re * aim + k.im * are, Ke)};
+  }
+}
+
 // NARROW: one lane per matrix, whole enumeration in the lane (small n), global rows.
 template <int N, int U, typename F>
 __global__ __launch_bounds__(256) void glynn_narrow_kernel(const cpx<F> *__restrict__ A,
@@ -561,6 +870,20 @@ constexpr int glynn_reg_rows() {
 #ifndef TB_GLYNN_F32_U
 #define TB_GLYNN_F32_U 3
 #endif
+// The paired FP32 kernel: 2 static Gray bits, row 1 in SGPRs (SR = 1) and row 2 in VGPRs,
+// so that it fits 3 waves per SIMD (<= 168 VGPRs; 137 at n = 20, 155 at n = 23). One
+// wave issues an FP32 instruction at most every ~4.2 cycles and the SIMD takes one every
+// 2 (measured with independent FMA chains), so at 2 waves every stall of either wave is
+// lost issue; 3 static bits with all rows in VGPRs need 217 VGPRs (2 waves) at n = 20.
+#ifndef TB_GLYNN_F32_PAIR_U
+#define TB_GLYNN_F32_PAIR_U 2
+#endif
+#ifndef TB_GLYNN_F32_PAIR_SR
+#define TB_GLYNN_F32_PAIR_SR 1
+#endif
+#ifndef TB_GLYNN_F32_PAIR_WPE
+#define TB_GLYNN_F32_PAIR_WPE 3
+#endif
 template <int N, typename F>
 constexpr int glynn_u() {
   if (sizeof(F) == 4)
@@ -585,7 +908,10 @@ constexpr int kWideMinN = 11;  // below this a lane owns a whole matrix
 #endif
 template <int N, typename F>
 constexpr bool glynn_paired() {
-  return sizeof(F) == 8 && N >= TB_GLYNN_PAIR_MIN_N && N <= TB_GLYNN_PAIR_MAX_N;
+  if constexpr (sizeof(F) == 8) return N >= TB_GLYNN_PAIR_MIN_N && N <= TB_GLYNN_PAIR_MAX_N;
+  // FP32 column sums: where the FP32 kernel is in use (larger n, run only to measure its
+  // error, keep the plain FP32 walk)
+  return N >= TB_GLYNN_PAIR_MIN_N && N <= TB_FP32_COLSUM_MAX_N;
 }
 // Static super-step bits of the paired walk and its rows in broadcast form (rows 1..B).
 template <int N>
@@ -600,10 +926,37 @@ constexpr int glynn_pair_b() {
 // Lower bound on C of a launch of n: the static bits, and for the paired walk its static
 // super-step bits plus the pair bit.
 template <int N, typename F>
+constexpr int glynn_pair_u_of() {
+  return sizeof(F) == 8 ? glynn_pair_u<N>() : TB_GLYNN_F32_PAIR_U;
+}
+// Values of row 2 the paired FP32 kernel also keeps in SGPRs, so that it fits 4 waves
+// per SIMD (<= 128 VGPRs): about 6n + 17 VGPRs without them, 2 fewer per value; up to 11
+// values (row 1 takes 2n SGPRs; n = 23 would need 14 and exceed the 102 SGPRs).
+#ifndef TB_GLYNN_F32_PAIR_S2_MAX
+#define TB_GLYNN_F32_PAIR_S2_MAX 11
+#endif
+template <int N>
+constexpr int glynn_f32_pair_s2() {
+  if (TB_GLYNN_F32_PAIR_SR != 1 || TB_GLYNN_F32_PAIR_U < 2) return 0;
+  constexpr int over = 6 * N + 17 - 128;
+  constexpr int need = over <= 0 ? 0 : (over + 1) / 2 + 1;  // one value of margin
+  return (over <= 0 || need > TB_GLYNN_F32_PAIR_S2_MAX) ? 0 : need;
+}
+// Waves per SIMD the launch is resident at (the plan's round size).
+template <int N, typename F>
+constexpr int glynn_wave_slots() {
+  if constexpr (sizeof(F) == 4 && glynn_paired<N, F>()) {
+    if constexpr (TB_GLYNN_F32_PAIR_SR == 1)
+      return (6 * N + 17 <= 128 || glynn_f32_pair_s2<N>() > 0) ? 4 : TB_GLYNN_F32_PAIR_WPE;
+    return TB_GLYNN_F32_PAIR_WPE;
+  }
+  return 2;
+}
+template <int N, typename F>
 constexpr int glynn_min_c() {
   constexpr int u = glynn_u<N, F>();
   if constexpr (glynn_paired<N, F>())
-    return u > glynn_pair_u<N>() + 1 ? u : glynn_pair_u<N>() + 1;
+    return u > glynn_pair_u_of<N, F>() + 1 ? u : glynn_pair_u_of<N, F>() + 1;
   return u;
 }
 constexpr int kMaxC = 12;      // Gray bits per lane chunk
@@ -618,7 +971,15 @@ struct GlynnPlan {
 // mid-size matrices still fill the GPU; C <= 12 bounds a lane's walk between re-seeds.
 // A tuned C (c_pref > 0) replaces that choice. Always: C >= U and a matrix gets at
 // least 4 waves (one workgroup). Re-seeding costs n^2 per 2^C steps.
-inline GlynnPlan glynn_plan(int n, int batch, int c_pref, int u = 2) {
+// Kernels resident at more than 2 waves per SIMD (wave_slots) compare the built-in C with
+// C - 1 and C - 2: the waves run in rounds of wave_slots x 1024 SIMDs (256 CUs), and a
+// smaller chunk can fill the last round better (16384 waves are 5.33 rounds of 3072 but
+// 10.67 rounds at twice the waves), against twice the re-seeding per code.
+constexpr int kPlanSimds = 1024;
+inline int glynn_plan_clamp(int C, int nb, int u) {
+  return std::min(std::max(std::min(C, kMaxC), u), nb - 8);
+}
+inline GlynnPlan glynn_plan(int n, int batch, int c_pref, int u = 2, int wave_slots = 2) {
   GlynnPlan p{};
   const int nb = n - 1;
   int C;
@@ -630,9 +991,24 @@ inline GlynnPlan glynn_plan(int n, int batch, int c_pref, int u = 2) {
     C = nb - 6 - want_bits;
     C = std::max(C, std::min(7, nb - 8));
   }
-  C = std::min(C, kMaxC);
-  C = std::max(C, u);
-  C = std::min(C, nb - 8);
+  C = glynn_plan_clamp(C, nb, u);
+  if (c_pref <= 0 && wave_slots > 2) {
+    double best = -1.0;
+    int best_c = C;
+    for (int c = C; c >= C - 2; --c) {
+      if (glynn_plan_clamp(c, nb, u) != c) continue;
+      const double waves = (double)batch * std::ldexp(1.0, nb - c - 6);
+      const double rounds = waves / ((double)kPlanSimds * wave_slots);
+      const double fill = rounds / std::ceil(rounds);
+      const double seed = (double)(n - 2) * 2.0 / (5.0 * std::ldexp(1.0, c));
+      const double score = fill * (1.0 - seed);
+      if (score > best + 1e-9) {
+        best = score;
+        best_c = c;
+      }
+    }
+    C = best_c;
+  }
   p.C = C;
   const int wave_bits = nb - p.C - 6;  // log2(waves needed at one chunk per lane), >= 2
   const int wpm_bits = std::min(wave_bits, kMaxWpmBits);
@@ -641,9 +1017,12 @@ inline GlynnPlan glynn_plan(int n, int batch, int c_pref, int u = 2) {
   return p;
 }
 
+// A: the matrices in the kernel's precision; A64 / mstride64: the same matrices in FP64
+// (the paired FP32 kernel forms its quotients from them).
 template <int N, typename F>
-inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, int batch, tb::cd *out,
-                              tb::cd *partial, int *handled, int c_pref, hipStream_t s) {
+inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, const tb::cd *A64,
+                              int64_t mstride64, int batch, tb::cd *out, tb::cd *partial,
+                              int *handled, int c_pref, hipStream_t s) {
   if constexpr (N < kWideMinN) {
     constexpr int U = (N - 1) < 2 ? (N - 1) : 2;
     hipLaunchKernelGGL((tb::glynn_narrow_kernel<N, U, F>), dim3((batch + 255) / 256), dim3(256), 0,
@@ -651,23 +1030,34 @@ inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, int batch, t
     (void)partial;
     (void)handled;
     (void)c_pref;
+    (void)A64;
+    (void)mstride64;
     return tbh::check_hip(hipGetLastError(), "glynn_narrow_kernel");
   } else {
     constexpr int U = glynn_u<N, F>();
     constexpr int CMIN = glynn_min_c<N, F>();
     static_assert(CMIN <= N - 9, "the static block must fit the smallest chunk (C <= n - 9)");
-    const GlynnPlan p = glynn_plan(N, batch, c_pref, CMIN);
+    const GlynnPlan p = glynn_plan(N, batch, c_pref, CMIN, glynn_wave_slots<N, F>());
     const unsigned blocks = (unsigned)batch * (unsigned)(p.wpm / 4);
     const int *skip = nullptr;
     if constexpr (glynn_paired<N, F>()) {
-      hipLaunchKernelGGL((tb::glynn_pair_kernel<N, glynn_pair_u<N>(), glynn_pair_b<N>()>),
-                         dim3(blocks), dim3(256), 0, s, (const tb::cpx<double> *)A, mstride, p.C,
-                         p.wpm, p.iters, partial, handled);
+      if constexpr (sizeof(F) == 8)
+        hipLaunchKernelGGL((tb::glynn_pair_kernel<N, glynn_pair_u<N>(), glynn_pair_b<N>()>),
+                           dim3(blocks), dim3(256), 0, s, (const tb::cpx<double> *)A64, mstride64,
+                           p.C, p.wpm, p.iters, partial, handled);
+      else
+        hipLaunchKernelGGL((tb::glynn_pair_kernel_f32<N, TB_GLYNN_F32_PAIR_U, TB_GLYNN_F32_PAIR_SR,
+                                                      glynn_f32_pair_s2<N>(),
+                                                      glynn_wave_slots<N, F>()>),
+                           dim3(blocks), dim3(256),
+                           0, s, A64, mstride64, p.C, p.wpm, p.iters, partial, handled);
       int32_t rc = tbh::check_hip(hipGetLastError(), "glynn_pair_kernel");
       if (rc) return rc;
       skip = handled;
     } else {
       (void)handled;
+      (void)A64;
+      (void)mstride64;
     }
     hipLaunchKernelGGL((tb::glynn_wide_kernel<N, U, glynn_reg_rows<N, F>(), F>), dim3(blocks), dim3(256), 0, s,
                        A, mstride, p.C, p.wpm, p.iters, partial, skip);
@@ -680,20 +1070,26 @@ inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, int batch, t
 }
 
 template <typename F, int N = 2>
-inline int32_t dispatch_glynn(int n, const tb::cpx<F> *A, int64_t mstride, int batch, tb::cd *out,
-                              tb::cd *partial, int *handled, int c_pref, hipStream_t s) {
+inline int32_t dispatch_glynn(int n, const tb::cpx<F> *A, int64_t mstride, const tb::cd *A64,
+                              int64_t mstride64, int batch, tb::cd *out, tb::cd *partial,
+                              int *handled, int c_pref, hipStream_t s) {
   if constexpr (N > TB_GLYNN_MAX_N) {
     return tbh::fail(TB_ERANGE, "glynn: n=%d exceeds TB_GLYNN_MAX_N=%d", n, TB_GLYNN_MAX_N);
   } else {
-    if (n == N) return launch_glynn_n<N, F>(A, mstride, batch, out, partial, handled, c_pref, s);
-    return dispatch_glynn<F, N + 1>(n, A, mstride, batch, out, partial, handled, c_pref, s);
+    if (n == N)
+      return launch_glynn_n<N, F>(A, mstride, A64, mstride64, batch, out, partial, handled,
+                                  c_pref, s);
+    return dispatch_glynn<F, N + 1>(n, A, mstride, A64, mstride64, batch, out, partial, handled,
+                                    c_pref, s);
   }
 }
 
-// `handled`: batch ints of workspace for the paired FP64 walk (unused by FP32).
+// `handled`: batch ints of workspace for the paired walks. The FP32 dispatch takes the
+// FP32 copy (A) and the FP64 originals (A64).
 int32_t glynn_dispatch_f64(int n, const tb::cpx<double> *A, int64_t mstride, int batch,
                            tb::cd *out, tb::cd *partial, int *handled, int c_pref, hipStream_t s);
-int32_t glynn_dispatch_f32(int n, const tb::cpx<float> *A, int64_t mstride, int batch,
-                           tb::cd *out, tb::cd *partial, int c_pref, hipStream_t s);
+int32_t glynn_dispatch_f32(int n, const tb::cpx<float> *A, int64_t mstride, const tb::cd *A64,
+                           int64_t mstride64, int batch, tb::cd *out, tb::cd *partial,
+                           int *handled, int c_pref, hipStream_t s);
 
 }  // namespace tbgl
