@@ -246,6 +246,40 @@ def test_guan_outer_batch_vs_oracle(T, oracle):
         GPURyserPermanentCalculator(U).compute_permanent_outer(S, np.array([[1, 0, 0, 0, 0, 0]]))
 
 
+def test_guan_outer_batch_uniform_side_and_cached_copies(T, oracle):
+    """The all-pairs path whose S side has one Guan code count (the 0/1 substates of a
+    collision-free input, as GCC-A passes them): whole T rows per orientation, T rows with
+    and without collisions so both launches run. Same values as the pairwise batch (the
+    same launches, per orientation, over the same pairs) and the oracle; an in-place edit
+    of the matrix between calls is seen (the device copies are keyed by content)."""
+    from itertools import combinations
+
+    from theboss_amd.permanent_calculators import GPURyserPermanentCalculator
+
+    rng = np.random.default_rng(16)
+    m, k = 8, 4
+    U = oracle.haar_unitary(m, 99)
+    S = np.zeros((0, m), dtype=np.int64)
+    for c in combinations(range(6), k):
+        row = np.zeros(m, dtype=np.int64)
+        row[list(c)] = 1
+        S = np.vstack([S, row])
+    Tm = np.array([np.bincount(rng.choice(m, k, replace=False), minlength=m) for _ in range(6)]
+                  + [np.bincount(rng.integers(0, m, k), minlength=m) for _ in range(6)]
+                  + [[2, 2, 0, 0, 0, 0, 0, 0]])
+    codes_t = np.prod(Tm + 1, axis=1)
+    assert (codes_t < 2 ** k).any() and (codes_t == 2 ** k).any()
+    calc = GPURyserPermanentCalculator(U)
+    got = calc.compute_permanent_outer(S, Tm).cpu().numpy()
+    pairs = calc.compute_permanent_batch_arrays(np.tile(S, (len(Tm), 1)), np.repeat(Tm, len(S), axis=0))
+    assert np.array_equal(got, pairs.reshape(len(Tm), len(S)))
+    ref = np.array([[oracle.perm_classic(U, s, t) for s in S] for t in Tm])
+    assert np.max(np.abs(got - ref)) <= 1e-12
+    calc.matrix *= 0.5  # in place: same object, new content
+    again = calc.compute_permanent_outer(S, Tm).cpu().numpy()
+    assert np.max(np.abs(again - ref * 0.5 ** k)) <= 1e-12
+
+
 def _occupations(rng, m, ks):
     return np.array([np.bincount(rng.integers(0, m, int(k)), minlength=m) for k in ks])
 

@@ -6,9 +6,13 @@ Base-class semantics follow theboss's BSPermanentCalculatorBase
 the matrix and states are stored by reference (None -> empty list), so in-place edits
 such as `calc.matrix *= sqrt(eta)` are seen by the next computation; the shape check
 raises AttributeError (:179-180). Each compute call gathers only the submatrix it needs
-on the host and ships that to the device; no device state is kept between calls, so the
-objects deepcopy and pickle like the reference's.
+on the host and ships that to the device; no device state is kept in the objects, so
+they deepcopy and pickle like the reference's. The all-pairs entry point keeps a small
+module-level cache of device copies keyed by content (the matrix and the fixed side of
+the pairs), which lock-step samplers pass unchanged on every round.
 """
+import hashlib
+from collections import OrderedDict
 from typing import List, Optional, Sequence
 
 import numpy as np
@@ -128,6 +132,29 @@ class GPUGlynnGrayFP32PermanentCalculator(GPUGlynnGrayPermanentCalculator):
     precision = L.TB_PREC_FP32_COLSUM
 
 
+_OUTER_CACHE: "OrderedDict" = OrderedDict()
+_OUTER_CACHE_MAX = 32
+
+
+def _content_key(tag: str, dev, a: np.ndarray):
+    a = np.ascontiguousarray(a)
+    return (tag, str(dev), a.dtype.str, a.shape, hashlib.blake2b(a.tobytes(), digest_size=16).digest())
+
+
+def _cached(key, make):
+    """Least-recently-used cache of device copies (content-keyed, so in-place edits of a
+    matrix or state array give a new key)."""
+    v = _OUTER_CACHE.get(key)
+    if v is None:
+        v = make()
+        _OUTER_CACHE[key] = v
+        while len(_OUTER_CACHE) > _OUTER_CACHE_MAX:
+            _OUTER_CACHE.popitem(last=False)
+    else:
+        _OUTER_CACHE.move_to_end(key)
+    return v
+
+
 class _GuanGPUCalculator(BSPermanentCalculatorBase):
     """Multiplicity-aware Guan-code calculators (Ryser, Chin-Huh) on the GPU."""
 
@@ -174,26 +201,46 @@ class _GuanGPUCalculator(BSPermanentCalculatorBase):
         T = D.require_gpu()
         dev = D.device()
         ns, nt = S.shape[0], Tm.shape[0]
-        codes_s = np.prod(S.astype(np.float64) + 1, axis=1)
+        codes_s, cs = _cached(_content_key("S", dev, S), lambda: (
+            np.prod(S.astype(np.float64) + 1, axis=1),
+            [D.to_device_i32(a, dev) for a in compact_batch(S)]))
         codes_t = np.prod(Tm.astype(np.float64) + 1, axis=1)
-        cs = [D.to_device_i32(a, dev) for a in compact_batch(S)]
         ct = [D.to_device_i32(a, dev) for a in compact_batch(Tm)]
         if max(cs[0].shape[1], ct[0].shape[1]) > min(L.TB_GUAN_MAX_COLS, L.TB_GUAN_MAX_ROWS) \
                 or max(codes_s.max(), codes_t.max()) > 2.0 ** 28:
             return None
         U = np.asarray(self._matrix, dtype=np.complex128)
+        Ud, Utd = _cached(_content_key("U", dev, U), lambda: (
+            D.to_device_c128(U, dev), D.to_device_c128(np.ascontiguousarray(U.T), dev)))
+
+        def run(transposed, j, i):
+            (em, eu), (pm, pu) = ((ct, cs) if transposed else (cs, ct))
+            ei, pi = (j, i) if transposed else (i, j)
+            return ops.guan_perm_batch(Utd if transposed else Ud, pm[pi], pu[pi], em[ei],
+                                       eu[ei], self.formula)
+
+        if np.all(codes_s == codes_s[0]):
+            # One code count on the S side (e.g. substates of a collision-free input): the
+            # orientation is a property of the T row, so the pairs of each orientation are
+            # whole rows of the result and no pair mask is formed.
+            res = T.empty((nt, ns), dtype=T.complex128, device=dev)
+            flip_rows = codes_t < codes_s[0]
+            for transposed in (False, True):
+                js = np.nonzero(flip_rows == transposed)[0]
+                if js.size == 0:
+                    continue
+                js_d = T.from_numpy(js.astype(np.int64)).to(dev)
+                j = js_d.repeat_interleave(ns)
+                i = T.arange(ns, device=dev).repeat(js.size)
+                res[js_d] = run(transposed, j, i).reshape(js.size, ns)
+            return res
         flip = T.from_numpy(codes_t[:, None] < codes_s[None, :]).to(dev).reshape(-1)
         res = T.empty(nt * ns, dtype=T.complex128, device=dev)
         for transposed in (False, True):
             idx = T.nonzero(flip == transposed).reshape(-1)
             if idx.numel() == 0:
                 continue
-            j, i = idx // ns, idx % ns
-            (em, eu), (pm, pu) = ((ct, cs) if transposed else (cs, ct))
-            ei, pi = (j, i) if transposed else (i, j)
-            M = U.T if transposed else U
-            res[idx] = ops.guan_perm_batch(D.to_device_c128(M, dev), pm[pi], pu[pi], em[ei],
-                                           eu[ei], self.formula)
+            res[idx] = run(transposed, idx // ns, idx % ns)
         return res.reshape(nt, ns)
 
     def compute_permanent(self) -> complex128:
