@@ -561,21 +561,25 @@ __device__ __forceinline__ void glynn_pair_lane_f32(const cpx<float> *S2, int C,
                                                     double &acc_im) {
   constexpr int NB = N - 2;  // Gray bits: row i + 1 at bit i
   const bool topbit_fixed = (C == NB);
-  // Row 1 (half of all flips, always a fixed direction) in SGPRs when SR = 1, as the
-  // scalar operand of the adds; rows SR + 1..U in VGPRs, except the first S2P values of
-  // row 2 (SR = 1), which also sit in SGPRs.
+  // Static row SR (row 1 flips on half of all pairs, row 2 on a quarter) in SGPRs when
+  // SR > 0, as the scalar operand of the adds (an FP32 instruction with an SGPR operand
+  // does not dual-issue, so the register it frees costs issue rate); the other static
+  // rows in VGPRs, except the first S2P values of row 2 beside an SGPR row 1.
+  static_assert(SR <= U, "the SGPR row is a static row");
   static_assert(S2P == 0 || (SR == 1 && U >= 2), "row 2 splits only beside an SGPR row 1");
+  constexpr int NR = U - (SR ? 1 : 0);  // static rows in VGPRs
   SRow<SR ? N : 0, float> sr;
-  if constexpr (SR) load_srow<N>(sr, S2 + N);
+  if constexpr (SR) load_srow<N>(sr, S2 + (size_t)SR * N);
   SRow<S2P, float> sr2;
   if constexpr (S2P > 0) load_srow<S2P>(sr2, S2 + 2 * N);
-  RegRows<N, U - SR, float> rr;
+  RegRows<N, NR, float> rr;  // rr[r]: static row index r, skipping SR - 1
 #pragma unroll
-  for (int r = 0; r < U - SR; ++r)
+  for (int r = 0; r < NR; ++r)
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       if (r == 0 && j < S2P) continue;
-      const cpx<float> v = S2[(size_t)(r + SR + 1) * N + j];
+      const int i = (SR && r >= SR - 1) ? r + 1 : r;
+      const cpx<float> v = S2[(size_t)(i + 1) * N + j];
       rr.re[r][j] = v.re;
       rr.im[r][j] = v.im;
     }
@@ -615,7 +619,7 @@ __device__ __forceinline__ void glynn_pair_lane_f32(const cpx<float> *S2, int C,
     // row i + 1: mode +1 / -1 (fixed direction) or 0 (direction sg); every call site
     // passes constants, so the branches fold away after unrolling
     auto do_flip = [&](int i, int mode, float sg) {
-      if (SR && i == 0) {
+      if (SR && i == SR - 1) {
         if constexpr (SR) {
           if (mode == 1)
             flip_reg<N, 1, float>(ur, ui, sr.re, sr.im, sg);
@@ -639,8 +643,8 @@ __device__ __forceinline__ void glynn_pair_lane_f32(const cpx<float> *S2, int C,
       }
       if (i < U) {
 #pragma unroll
-        for (int r = 0; r < U - SR; ++r)
-          if (r + SR == i) {
+        for (int r = 0; r < NR; ++r)
+          if (((SR && r >= SR - 1) ? r + 1 : r) == i) {
             if (mode == 1)
               flip_reg<N, 1, float>(ur, ui, rr.re[r], rr.im[r], sg);
             else if (mode == -1)
@@ -870,11 +874,14 @@ constexpr int glynn_reg_rows() {
 #ifndef TB_GLYNN_F32_U
 #define TB_GLYNN_F32_U 3
 #endif
-// The paired FP32 kernel: 2 static Gray bits, row 1 in SGPRs (SR = 1) and row 2 in VGPRs,
-// so that it fits 3 waves per SIMD (<= 168 VGPRs; 137 at n = 20, 155 at n = 23). One
-// wave issues an FP32 instruction at most every ~4.2 cycles and the SIMD takes one every
-// 2 (measured with independent FMA chains), so at 2 waves every stall of either wave is
-// lost issue; 3 static bits with all rows in VGPRs need 217 VGPRs (2 waves) at n = 20.
+// The paired FP32 kernel: 2 static Gray bits, static row SR in SGPRs (row 1 by default)
+// and the other in VGPRs. A SIMD issues two FP32 instructions per 4-cycle slot only from
+// two different waves, so the kernel wants 4 resident waves (<= 128 VGPRs): row 1 in
+// SGPRs and, at n = 19..21, the first values of row 2 too (glynn_f32_pair_s2) give 4 waves
+// up to n = 21 and 3 at n = 22, 23. 3 static bits with every row in VGPRs need 217 VGPRs
+// (2 waves) at n = 20. SR = 2 (row 2 in SGPRs, so the row-1 flips dual-issue) fits 4
+// waves at every n and measured the same time at n = 20 and 5% slower at n = 23: at
+// n = 20 the launch runs the clock down to 1.9 GHz whichever rows dual-issue.
 #ifndef TB_GLYNN_F32_PAIR_U
 #define TB_GLYNN_F32_PAIR_U 2
 #endif
@@ -948,6 +955,7 @@ constexpr int glynn_wave_slots() {
   if constexpr (sizeof(F) == 4 && glynn_paired<N, F>()) {
     if constexpr (TB_GLYNN_F32_PAIR_SR == 1)
       return (6 * N + 17 <= 128 || glynn_f32_pair_s2<N>() > 0) ? 4 : TB_GLYNN_F32_PAIR_WPE;
+    if constexpr (TB_GLYNN_F32_PAIR_SR == 2) return 4;
     return TB_GLYNN_F32_PAIR_WPE;
   }
   return 2;
