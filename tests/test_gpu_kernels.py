@@ -127,8 +127,9 @@ def test_glynn_fp32_paired_walk_and_plain_fallback_in_one_batch(T, ops, oracle, 
     formed in FP64 from the FP64 matrix, then FP32 sums and products) and leave matrices
     without a row clear of small entries to the plain FP32 walk. The same mix of blocks as
     the FP64 test, minus the 2^-30 scaling (the plain FP32 walk keeps no exponent), each
-    within 3e-4 relative of the oracle (the FP32 error table's largest measured error at
-    these n is 4.8e-4 at n = 23, over 64 Haar blocks; these blocks sit far below it)."""
+    within 3e-4 relative of the oracle (over 64 Haar blocks the largest measured error at
+    these n is 8.1e-5 with the paired walk and 4.8e-4 with the plain one, both at n = 23;
+    these blocks sit far below them)."""
     from theboss_amd import _lib
 
     rng = np.random.default_rng(400 + n)
@@ -159,6 +160,42 @@ def test_glynn_fp32_paired_walk_and_plain_fallback_in_one_batch(T, ops, oracle, 
     print(f"fp32 paired/plain n={n}: rel err per block {np.array2string(err, precision=2)}")
     assert err.max() <= 3e-4, (n, err)
     assert not np.array_equal(got, f64)  # FP32 kernels ran, not the FP64 ones
+
+
+def test_glynn_fp32_batch_whose_plan_counts_four_waves(T, ops, oracle):
+    """n = 14, 200 matrices: the paired FP32 kernel (4 waves per SIMD) plans smaller
+    chunks, i.e. more partials per matrix, than a 2- or 3-wave plan would, and the
+    workspace must hold them: the partials sit before the FP32 copy of the matrices that
+    the plain FP32 walk reads for the matrices the paired kernel leaves (every 7th here,
+    a zero in every row). Through both entry points, against the oracle."""
+    from theboss_amd import _device as D
+    from theboss_amd import _lib
+
+    n, B = 14, 200
+    rng = np.random.default_rng(1400)
+    mats = []
+    for b in range(B):
+        A = oracle.haar_unitary(2 * n, 14000 + b)[:n, :n]
+        if b % 7 == 3:
+            A = A.copy()
+            A[np.arange(n), rng.integers(0, n, n)] = 0.0
+        mats.append(A)
+    As = np.stack(mats)
+    ref = oracle.perm_glynn_batch(As, 1)
+    Ad = dev_c(T, As)
+    got = ops.glynn_batch(Ad, _lib.TB_PREC_FP32_COLSUM).cpu().numpy()
+    err = np.abs(got - ref) / np.abs(ref)
+    plain = np.arange(B) % 7 == 3
+    assert err[~plain].max() <= 3e-4, (int(err.argmax()), err.max())
+    # the plain walk's blocks (zeros, more cancellation): a bound that an overwritten
+    # FP32 copy of the matrix would miss by orders of magnitude
+    assert err[plain].max() <= 1e-2, (int(err.argmax()), err.max())
+    out = T.empty(B, dtype=T.complex128, device=Ad.device)
+    _lib.check(_lib.load().tb_glynn_batch_noalloc(D.ptr(Ad), n, B, n * n,
+                                                  _lib.TB_PREC_FP32_COLSUM, D.ptr(out),
+                                                  D.stream_ptr(Ad.device)))
+    T.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), got)
 
 
 @pytest.mark.parametrize("n", [39, 40, 41])

@@ -5,8 +5,8 @@
 namespace tbgl {
 int32_t glynn_dispatch_f32(int n, const tb::cpx<float> *A, int64_t mstride, const tb::cd *A64,
                            int64_t mstride64, int batch, tb::cd *out, tb::cd *partial,
-                           int *handled, int c_pref, hipStream_t s) {
+                           int *handled, int wpm_cap, int c_pref, hipStream_t s) {
   return dispatch_glynn<float>(n, A, mstride, A64, mstride64, batch, out, partial, handled,
-                               c_pref, s);
+                               wpm_cap, c_pref, s);
 }
 }  // namespace tbgl

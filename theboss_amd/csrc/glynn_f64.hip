@@ -6,9 +6,10 @@
 
 namespace tbgl {
 int32_t glynn_dispatch_f64(int n, const tb::cpx<double> *A, int64_t mstride, int batch,
-                           tb::cd *out, tb::cd *partial, int *handled, int c_pref, hipStream_t s) {
+                           tb::cd *out, tb::cd *partial, int *handled, int wpm_cap, int c_pref,
+                           hipStream_t s) {
   return dispatch_glynn<double>(n, A, mstride, (const tb::cd *)A, mstride, batch, out, partial,
-                                handled, c_pref, s);
+                                handled, wpm_cap, c_pref, s);
 }
 }  // namespace tbgl
 
@@ -18,15 +19,20 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // The "handled" flags of the paired walk (one int per matrix) ahead of the partials.
 size_t tbgl_flag_bytes(int batch) { return align256((size_t)batch * sizeof(int)); }
 
+// Partials per matrix the workspace holds: the largest over every plan a launch of n can
+// make, i.e. every lower bound on C (the static bits, 2..4, and the paired walks' static
+// bits plus one) and every residency the plan may count (2, 3 or 4 waves per SIMD).
+int tbgl_partial_wpm(int n, int batch, int c_pref) {
+  int wpm = 0;
+  for (int u : {2, 3, 4, tbgl::glynn_u_max(n)})
+    for (int slots : {2, 3, 4}) wpm = std::max(wpm, tbgl::glynn_plan(n, batch, c_pref, u, slots).wpm);
+  return wpm;
+}
+
 size_t tbgl_partial_bytes(int n, int batch, int c_pref) {
   if (n < tbgl::kWideMinN) return 0;
-  // large enough for the plan of either precision and either residency (their lower
-  // bounds on C may differ, and a larger bound only shrinks wpm)
-  int wpm = 0;
-  for (int u : {2, tbgl::glynn_u_max(n)})
-    for (int slots : {2, TB_GLYNN_F32_PAIR_WPE})
-      wpm = std::max(wpm, tbgl::glynn_plan(n, batch, c_pref, u, slots).wpm);
-  return tbgl_flag_bytes(batch) + align256((size_t)batch * wpm * sizeof(tb::cd));
+  return tbgl_flag_bytes(batch) +
+         align256((size_t)batch * tbgl_partial_wpm(n, batch, c_pref) * sizeof(tb::cd));
 }
 
 // The precision a call runs in: FP32 column sums only up to TB_FP32_COLSUM_MAX_N unless
@@ -83,9 +89,10 @@ extern "C" int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t bat
   char *ws = (char *)workspace;
   int *handled = (int *)ws;
   tb::cd *partial = ws ? (tb::cd *)(ws + tbgl_flag_bytes(batch)) : nullptr;
+  const int wpm_cap = n >= tbgl::kWideMinN ? tbgl_partial_wpm(n, batch, c_pref) : 0;
   if (!runs_fp32(n, precision))
     return tbgl::glynn_dispatch_f64(n, (const tb::cpx<double> *)A, matrix_stride, batch, o,
-                                    partial, handled, c_pref, s);
+                                    partial, handled, wpm_cap, c_pref, s);
   // FP32 column sums: convert once into the tail of the workspace.
   float *A32 = (float *)(ws + tbgl_partial_bytes(n, batch, c_pref));
   const int64_t total = (int64_t)batch * n * n;
@@ -95,7 +102,7 @@ extern "C" int32_t tb_glynn_batch(const tb_complex128 *A, int32_t n, int32_t bat
   if (rc) return rc;
   return tbgl::glynn_dispatch_f32(n, (const tb::cpx<float> *)A32, (int64_t)n * n,
                                   (const tb::cd *)A, matrix_stride, batch, o, partial, handled,
-                                  c_pref, s);
+                                  wpm_cap, c_pref, s);
 }
 
 // ------------------------------------------------------------------------------------

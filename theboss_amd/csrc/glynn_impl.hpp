@@ -1027,16 +1027,19 @@ inline GlynnPlan glynn_plan(int n, int batch, int c_pref, int u = 2, int wave_sl
 
 // A: the matrices in the kernel's precision; A64 / mstride64: the same matrices in FP64
 // (the paired FP32 kernel forms its quotients from them).
+// wpm_cap: partials per matrix the workspace holds; a plan that needs more fails with
+// TB_EWORKSPACE instead of writing past them.
 template <int N, typename F>
 inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, const tb::cd *A64,
                               int64_t mstride64, int batch, tb::cd *out, tb::cd *partial,
-                              int *handled, int c_pref, hipStream_t s) {
+                              int *handled, int wpm_cap, int c_pref, hipStream_t s) {
   if constexpr (N < kWideMinN) {
     constexpr int U = (N - 1) < 2 ? (N - 1) : 2;
     hipLaunchKernelGGL((tb::glynn_narrow_kernel<N, U, F>), dim3((batch + 255) / 256), dim3(256), 0,
                        s, A, mstride, batch, out);
     (void)partial;
     (void)handled;
+    (void)wpm_cap;
     (void)c_pref;
     (void)A64;
     (void)mstride64;
@@ -1046,6 +1049,10 @@ inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, const tb::cd
     constexpr int CMIN = glynn_min_c<N, F>();
     static_assert(CMIN <= N - 9, "the static block must fit the smallest chunk (C <= n - 9)");
     const GlynnPlan p = glynn_plan(N, batch, c_pref, CMIN, glynn_wave_slots<N, F>());
+    if (p.wpm > wpm_cap)
+      return tbh::fail(TB_EWORKSPACE,
+                       "glynn: the plan for n=%d batch=%d needs %d partials per matrix, the "
+                       "workspace holds %d", N, batch, p.wpm, wpm_cap);
     const unsigned blocks = (unsigned)batch * (unsigned)(p.wpm / 4);
     const int *skip = nullptr;
     if constexpr (glynn_paired<N, F>()) {
@@ -1080,24 +1087,25 @@ inline int32_t launch_glynn_n(const tb::cpx<F> *A, int64_t mstride, const tb::cd
 template <typename F, int N = 2>
 inline int32_t dispatch_glynn(int n, const tb::cpx<F> *A, int64_t mstride, const tb::cd *A64,
                               int64_t mstride64, int batch, tb::cd *out, tb::cd *partial,
-                              int *handled, int c_pref, hipStream_t s) {
+                              int *handled, int wpm_cap, int c_pref, hipStream_t s) {
   if constexpr (N > TB_GLYNN_MAX_N) {
     return tbh::fail(TB_ERANGE, "glynn: n=%d exceeds TB_GLYNN_MAX_N=%d", n, TB_GLYNN_MAX_N);
   } else {
     if (n == N)
       return launch_glynn_n<N, F>(A, mstride, A64, mstride64, batch, out, partial, handled,
-                                  c_pref, s);
+                                  wpm_cap, c_pref, s);
     return dispatch_glynn<F, N + 1>(n, A, mstride, A64, mstride64, batch, out, partial, handled,
-                                    c_pref, s);
+                                    wpm_cap, c_pref, s);
   }
 }
 
-// `handled`: batch ints of workspace for the paired walks. The FP32 dispatch takes the
-// FP32 copy (A) and the FP64 originals (A64).
+// `handled`: batch ints of workspace for the paired walks; `wpm_cap`: partials per matrix
+// after them. The FP32 dispatch takes the FP32 copy (A) and the FP64 originals (A64).
 int32_t glynn_dispatch_f64(int n, const tb::cpx<double> *A, int64_t mstride, int batch,
-                           tb::cd *out, tb::cd *partial, int *handled, int c_pref, hipStream_t s);
+                           tb::cd *out, tb::cd *partial, int *handled, int wpm_cap, int c_pref,
+                           hipStream_t s);
 int32_t glynn_dispatch_f32(int n, const tb::cpx<float> *A, int64_t mstride, const tb::cd *A64,
                            int64_t mstride64, int batch, tb::cd *out, tb::cd *partial,
-                           int *handled, int c_pref, hipStream_t s);
+                           int *handled, int wpm_cap, int c_pref, hipStream_t s);
 
 }  // namespace tbgl
