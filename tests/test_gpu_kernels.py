@@ -198,6 +198,48 @@ def test_glynn_fp32_batch_whose_plan_counts_four_waves(T, ops, oracle):
     assert np.array_equal(out.cpu().numpy(), got)
 
 
+@pytest.mark.parametrize("n", [11, 13, 14, 16, 19, 21, 22, 23, 26])
+def test_glynn_chunk_plans_across_batch_sizes(T, ops, oracle, n):
+    """The chunk plan depends on n, the batch and the residency the launch counts (2 waves
+    per SIMD for FP64, 3-4 for the paired FP32 kernel); every plan must fit the workspace
+    and give the same permanents. Batches of 1, 5, 37 and 200 (fewer at large n) with
+    every 5th matrix left to the plain walk (a zero in every row): FP64 against the same
+    matrices one per call (a different plan) to 1e-12, FP32 against FP64 (3e-4 where the
+    paired kernel ran, 1e-2 on the plain walk's blocks), and the no-alloc entry point
+    bit-identical to the workspace one."""
+    from theboss_amd import _device as D
+    from theboss_amd import _lib
+
+    rng = np.random.default_rng(7000 + n)
+    for B in ((1, 5, 37, 200) if n <= 21 else (1, 5, 37)):
+        mats = []
+        for b in range(B):
+            A = oracle.haar_unitary(2 * n, 70000 + 1000 * n + b)[:n, :n]
+            if b % 5 == 4:
+                A = A.copy()
+                A[np.arange(n), rng.integers(0, n, n)] = 0.0
+            mats.append(A)
+        As = np.stack(mats)
+        Ad = dev_c(T, As)
+        f64 = ops.glynn_batch(Ad, _lib.TB_PREC_FP64).cpu().numpy()
+        for b in sorted({0, B // 2, B - 1}):
+            one = ops.glynn_batch(dev_c(T, As[b:b + 1]), _lib.TB_PREC_FP64).cpu().numpy()[0]
+            assert abs(one - f64[b]) <= 1e-12 * abs(f64[b]), (n, B, b)
+        for prec in (_lib.TB_PREC_FP64, _lib.TB_PREC_FP32_COLSUM):
+            got = ops.glynn_batch(Ad, prec).cpu().numpy()
+            out = T.empty(B, dtype=T.complex128, device=Ad.device)
+            _lib.check(_lib.load().tb_glynn_batch_noalloc(D.ptr(Ad), n, B, n * n, prec,
+                                                          D.ptr(out), D.stream_ptr(Ad.device)))
+            T.cuda.synchronize()
+            assert np.array_equal(out.cpu().numpy(), got), (n, B, prec)
+            if prec == _lib.TB_PREC_FP32_COLSUM and n <= 23:
+                err = np.abs(got - f64) / np.abs(f64)
+                plain = np.arange(B) % 5 == 4
+                assert err[~plain].max() <= 3e-4, (n, B, err.max())
+                if plain.any():
+                    assert err[plain].max() <= 1e-2, (n, B, err.max())
+
+
 @pytest.mark.parametrize("n", [39, 40, 41])
 def test_glynn_fp64_rank_one_large(T, ops, oracle, n):
     """Near the largest supported size (2^38..2^40 Gray steps; n = 39 is the largest with
