@@ -30,6 +30,26 @@ def test_compact_batch_order_and_padding():
     assert modes[2][0] == 0 and list(mult[2]) == [3, 0]
 
 
+def test_device_compaction_equals_compact_batch():
+    """The scatter-based compaction of the all-pairs launch (run here on CPU tensors)
+    gives compact_batch's arrays: modes in mode order, padded with mode 0 of multiplicity
+    0, width the largest occupied-mode count; including all-empty rows."""
+    import torch
+
+    from theboss_amd.permanent_calculators.gpu_permanent_calculators import _compact_on_device
+    from theboss_amd.states import compact_batch
+
+    rng = np.random.default_rng(0)
+    cases = [np.array([[0, 2, 0, 1], [0, 0, 0, 0], [3, 0, 0, 0]]), np.zeros((2, 3), dtype=np.int64)]
+    for _ in range(100):
+        B, m, k = (int(x) for x in (rng.integers(1, 40), rng.integers(1, 22), rng.integers(1, 11)))
+        cases.append(np.array([np.bincount(rng.integers(0, m, k), minlength=m) for _ in range(B)]))
+    for S in cases:
+        modes, mult = compact_batch(S)
+        dm, du = _compact_on_device(torch, S, "cpu")
+        assert np.array_equal(modes, dm.numpy()) and np.array_equal(mult, du.numpy())
+
+
 def test_effective_matrix_matches_oracle(oracle):
     from theboss_amd.states import effective_matrix
 

@@ -141,6 +141,22 @@ def _content_key(tag: str, dev, a: np.ndarray):
     return (tag, str(dev), a.dtype.str, a.shape, hashlib.blake2b(a.tobytes(), digest_size=16).digest())
 
 
+def _compact_on_device(T, states: np.ndarray, dev):
+    """compact_batch on the device: (modes, multiplicities) B x W int32 tensors, W the
+    largest number of occupied modes of a row (counted on the host), each row's occupied
+    modes in mode order, padded with mode 0 of multiplicity 0. One upload and a scatter,
+    no synchronisation."""
+    W = max(1, int((states > 0).sum(axis=1).max())) if len(states) else 1
+    t = T.from_numpy(np.ascontiguousarray(states, dtype=np.int32)).to(dev)
+    B, m = t.shape
+    occ = t > 0
+    slot = T.where(occ, T.cumsum(occ, dim=1, dtype=T.int64) - 1, T.full_like(t, W, dtype=T.int64))
+    cols = T.arange(m, device=dev, dtype=T.int32).expand(B, m)
+    modes = T.zeros((B, W + 1), dtype=T.int32, device=dev).scatter_(1, slot, cols)
+    mult = T.zeros((B, W + 1), dtype=T.int32, device=dev).scatter_(1, slot, t)
+    return [modes[:, :W].contiguous(), mult[:, :W].contiguous()]
+
+
 def _cached(key, make):
     """Least-recently-used cache of device copies (content-keyed, so in-place edits of a
     matrix or state array give a new key)."""
@@ -205,7 +221,7 @@ class _GuanGPUCalculator(BSPermanentCalculatorBase):
             np.prod(S.astype(np.float64) + 1, axis=1),
             [D.to_device_i32(a, dev) for a in compact_batch(S)]))
         codes_t = np.prod(Tm.astype(np.float64) + 1, axis=1)
-        ct = [D.to_device_i32(a, dev) for a in compact_batch(Tm)]
+        ct = _compact_on_device(T, Tm, dev)
         if max(cs[0].shape[1], ct[0].shape[1]) > min(L.TB_GUAN_MAX_COLS, L.TB_GUAN_MAX_ROWS) \
                 or max(codes_s.max(), codes_t.max()) > 2.0 ** 28:
             return None
