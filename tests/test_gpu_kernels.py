@@ -494,9 +494,9 @@ def test_submatrices_collision_inputs_vs_oracle(T, oracle):
         assert np.max(np.abs(got - ref)) <= 1e-12, (s, t)
 
 
-@pytest.mark.parametrize("k", [12, 16, 21])
+@pytest.mark.parametrize("k", [12, 16, 21, 22, 26])
 def test_sweep_paired_and_plain_items_in_one_batch(T, ops, oracle, k):
-    """Sweeps of 8 <= k <= 21 whose chunks hold at least 2^4 codes (here: 7 items, chunks
+    """Sweeps of 8 <= k <= 26 whose chunks hold at least 2^4 codes (here: 7 items, chunks
     of 2^5) run the paired walk (particle 1 folded into two products per pair of Gray
     codes) on every item with a particle whose column entries all exceed
     2^-16 of the item's largest entry, the plain walk on the others, in one launch. The

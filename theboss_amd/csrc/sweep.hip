@@ -233,9 +233,13 @@ inline size_t sweep_pair_lds_bytes(int k, int c) {
          sizeof(double) * (size_t)(2 * k) + sizeof(tb::cd) + 2 * sizeof(int);
 }
 // Sizes that run the paired sweep (with a chunk holding the static block plus the pair
-// bit): up to the largest k that keeps 2 waves per SIMD without spills.
+// bit). Up to k = 21 it fits 256 VGPRs; k = 22..26 spill a few values of the per-task
+// setup and per-chunk code (not the block loop) and still beat the plain walk (uniform
+// loss, m = 80, n = 40: 4.5% more samples/s than pairing up to 21). From k = 27 the
+// compiler parks the broadcast columns in AGPRs and reloads them right before their DPP
+// reads, a hazard (tests/test_dpp_hazards.py), so larger k keep the plain walk.
 #ifndef TB_SWEEP_PAIR_MAX_K
-#define TB_SWEEP_PAIR_MAX_K 21
+#define TB_SWEEP_PAIR_MAX_K 26
 #endif
 template <int K>
 constexpr bool sweep_paired() {
