@@ -53,7 +53,7 @@ __global__ __launch_bounds__(64) TB_SWEEP_ATTR(K) void sweep_kernel(
 // walk. The swap is undone when the minors are written. LDS: Vs | Ed | inv[R] | pmin[K] |
 // pmax[K] | Kc, exponent, pick.
 template <int K, int U>
-__global__ __launch_bounds__(64) TB_SWEEP_ATTR(K) void sweep_pair_kernel(
+__global__ __launch_bounds__(64) TB_SWEEP_PAIR_ATTR(K) void sweep_pair_kernel(
     const cd *__restrict__ Vt, int ldv, int64_t item_stride, const int *__restrict__ item_map,
     const int *__restrict__ item_count, int items, int c, int wpi, int iters,
     cd *__restrict__ partial) {
@@ -233,13 +233,13 @@ inline size_t sweep_pair_lds_bytes(int k, int c) {
          sizeof(double) * (size_t)(2 * k) + sizeof(tb::cd) + 2 * sizeof(int);
 }
 // Sizes that run the paired sweep (with a chunk holding the static block plus the pair
-// bit). Up to k = 21 it fits 256 VGPRs; k = 22..26 spill a few values of the per-task
-// setup and per-chunk code (not the block loop) and still beat the plain walk (uniform
-// loss, m = 80, n = 40: 4.5% more samples/s than pairing up to 21). From k = 27 the
-// compiler parks the broadcast columns in AGPRs and reloads them right before their DPP
-// reads, a hazard (tests/test_dpp_hazards.py), so larger k keep the plain walk.
+// bit), at 2 waves per SIMD: up to k = 21 with five particles in broadcast form, from
+// k = 22 with the three static ones (196..249 VGPRs, no spills). Five broadcast particles
+// spill from k = 22, and at one wave per SIMD the compiler parks broadcast columns in
+// AGPRs and reloads them right before their DPP reads, a hazard that
+// tests/test_dpp_hazards.py looks for. k = 31 is the largest broadcast-column sweep.
 #ifndef TB_SWEEP_PAIR_MAX_K
-#define TB_SWEEP_PAIR_MAX_K 26
+#define TB_SWEEP_PAIR_MAX_K 31
 #endif
 template <int K>
 constexpr bool sweep_paired() {

@@ -34,6 +34,12 @@
 #ifndef TB_SWEEP_ATTR
 #define TB_SWEEP_ATTR(K) __attribute__((amdgpu_waves_per_eu((K) <= 26 ? 2 : 1)))
 #endif
+// The paired sweep keeps 2 waves per SIMD at every k it runs at: at one wave the compiler
+// would park broadcast columns in AGPRs (a DPP hazard), and FP64 issue at one wave is
+// latency-bound.
+#ifndef TB_SWEEP_PAIR_ATTR
+#define TB_SWEEP_PAIR_ATTR(K) __attribute__((amdgpu_waves_per_eu(2)))
+#endif
 #ifndef TB_STEP_FENCE
 #define TB_STEP_FENCE() __builtin_amdgcn_sched_barrier(0)
 #endif
@@ -344,10 +350,17 @@ __device__ __forceinline__ cd sweep_wave(const cd *Vs, cd *Ed, int lane, int c, 
 // ------------------------------------------------------------------------------------
 constexpr int kSweepPairMinK = 8;
 
-// Particles 2..NU+1 in broadcast form.
+// Particles 2..NU+1 in broadcast form: the static ones and the two most frequent
+// block-start ones up to k = TB_SWEEP_PAIR_NU5_MAX_K, the static ones alone above (their
+// VGPRs go to the longer column sums; with five the kernel spills from k = 22). Uniform
+// loss m = 80, n = 40: 4.8% more samples/s than five up to k = 26 and the plain walk above.
+#ifndef TB_SWEEP_PAIR_NU5_MAX_K
+#define TB_SWEEP_PAIR_NU5_MAX_K 21
+#endif
 template <int R>
 struct ColP {
-  static constexpr int NU = (kSweepU + 2 < R - 1) ? kSweepU + 2 : R - 1;
+  static constexpr int NUW = (R + 1 <= TB_SWEEP_PAIR_NU5_MAX_K) ? kSweepU + 2 : kSweepU;
+  static constexpr int NU = (NUW < R - 1) ? NUW : R - 1;
   Bcast<R> b[NU];
   double one;
 };
