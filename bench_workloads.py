@@ -126,6 +126,17 @@ def sweep_small(checker):
     return {"workload": "C&C minors sweep, k=20, batches 1..4096 (built-in plans)", "results": rows}
 
 
+def grad_executed_flops(n):
+    """Flops the single-sweep gradient executes per n x n matrix: per Gray step of a group of
+    G columns a row flip (2n), n + 2G - 5 complex multiplies (6 each), G running sums (2
+    each) and G corrector FMAs (4 each), 8n + 18G - 30 flops, over 2^(n-1) steps; groups of
+    8 columns, the last 1..4 columns (n mod 8) in one group of 4, else ceil(n/8) groups of 8
+    (the last overlapping its predecessor)."""
+    full, rest = divmod(n, 8)
+    groups = [8] * full + ([4] if 0 < rest <= 4 else [8] if rest else [])
+    return 2 ** (n - 1) * sum(8 * n + 18 * g - 30 for g in groups)
+
+
 def glynn_flops(n):
     """Executed flops of one n x n Glynn permanent: 2^(n-1) Gray steps, each n complex
     column adds (2n), n - 1 complex multiplies (6 each) and the signed accumulation (2),
@@ -332,11 +343,7 @@ def grad(checker, cpu=True, tune=True):
     # All n^2 minors by one sweep: 2^(n-1) (2n^2 + 20n) (n column adds, ~3n multiplies for
     # leave-one-out products, n^2 signed accumulations per step).
     f_grad = B * 2 ** (n - 1) * (2 * n * n + 20 * n)
-    # Executed: ceil(n/G) sweeps over groups of G = 8 columns, each step a row flip (2n),
-    # n + 2G - 5 complex multiplies (6 each), G running sums (2 each) and G corrector FMAs
-    # (4 each): 8n + 18G - 30 flops.
-    G = 8
-    f_bwd_exec = B * ((n + G - 1) // G) * 2 ** (n - 1) * (8 * n + 18 * G - 30)
+    f_bwd_exec = B * grad_executed_flops(n)
     return {"workload": "permanent + gradient, 64 x n=20",
             "autograd_fwd_bwd_ms": t_pass * 1e3,
             "cpu_baseline": cpu_rec,
@@ -351,8 +358,10 @@ def grad(checker, cpu=True, tune=True):
             "note": "autograd pass: the forward of permanent() on a tensor that requires grad "
                     "runs the gradient launch, which returns perm and all n^2 minors (one Gray "
                     "sweep per group of 8 columns); the backward scales the kept minors. "
-                    "executed: the flops the sweeps perform, ceil(n/8) 2^(n-1) (8n + 114) per "
-                    "matrix (the FLOP counter reads ~1.2x this: FMAs with a +-1 or +-2 operand "
+                    "executed: the flops the sweeps perform, per matrix 2^(n-1) (8n + 114) for "
+                    "each group of 8 columns and 2^(n-1) (8n + 42) for the group of 4 that takes "
+                    "the last n mod 8 <= 4 columns (n = 20: 2 of 8 and 1 of 4) "
+                    "(the FLOP counter reads ~1.2x this: FMAs with a +-1 or +-2 operand "
                     "count 2); _kernels: over the gradient launches alone, without the autograd "
                     "host work. effective: algorithm-equivalent flops, a Glynn forward (8n - 4) 2^(n-1) "
                     "plus the all-minors count 2^(n-1)(2n^2 + 20n) per matrix, which charges "

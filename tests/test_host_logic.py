@@ -183,6 +183,14 @@ def test_bench_flop_model():
     # one sample: sum_k [2^(k-1)(10k-12) + 8 m k]
     expect = sum(2 ** (k - 1) * (10 * k - 12) + 8 * 400 * k for k in range(1, 21))
     assert bench.sample_flops(20, 400) == expect
+    import bench_workloads as W
+
+    # gradient: groups of 8 columns (8n + 114 per step), the last n mod 8 <= 4 columns in
+    # one group of 4 (8n + 42), else an overlapping group of 8
+    assert W.grad_executed_flops(20) == 2 ** 19 * (2 * 274 + 202)
+    assert W.grad_executed_flops(24) == 2 ** 23 * 3 * 306
+    assert W.grad_executed_flops(21) == 2 ** 20 * 3 * 282
+    assert W.grad_executed_flops(17) == 2 ** 16 * (2 * 250 + 178)
 
 
 def test_state_enumeration_matches_oracle(oracle):

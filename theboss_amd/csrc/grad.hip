@@ -36,8 +36,10 @@
 // lanes, position): per slot q, column 4q + grad_row_col(r) of output rows p and 16 + p.
 //
 // Columns. The item (matrix, group g) stages its matrix in LDS with the group's columns
-// last, so the product structure is static. The last group starts at n - G and may
-// overlap its predecessor; the reduction takes each column from its first group.
+// last, so the product structure is static. With 8-column groups and 1..4 columns left
+// over, those come from one 4-column group per matrix (columns n-4..n-1, a second launch);
+// otherwise the last group starts at n - G and may overlap its predecessor. The reduction
+// takes each column from its first group.
 #include <algorithm>
 
 #include "flip_common.hpp"
@@ -81,11 +83,24 @@ template <int N>
 constexpr int grad_g() {
   return (N >= TB_GRAD_G8_MIN_N && N <= TB_GRAD_G8_MAX_N) ? 8 : 4;
 }
+template <int N, int G>
+constexpr int grad_u_of() {
+  return G == 8 ? TB_GRAD_G8_U : (N <= TB_GRAD_U4_MAX_N ? 4 : 3);
+}
 template <int N>
 constexpr int grad_u() {
-  return grad_g<N>() == 8 ? TB_GRAD_G8_U : (N <= TB_GRAD_U4_MAX_N ? 4 : 3);
+  return grad_u_of<N, grad_g<N>()>();
 }
 inline int grad_g_of(int n) { return (n >= TB_GRAD_G8_MIN_N && n <= TB_GRAD_G8_MAX_N) ? 8 : 4; }
+// 8-column groups with 1..4 columns left over: those go to one 4-column group (a second
+// launch) instead of an 8-column group that overlaps its predecessor, which would redo
+// 4..7 columns (n = 20: 2 x 274 + 202 flops per Gray step instead of 3 x 274).
+#ifndef TB_GRAD_TAIL4
+#define TB_GRAD_TAIL4 1
+#endif
+inline bool grad_tail4(int n) {
+  return TB_GRAD_TAIL4 && grad_g_of(n) == 8 && n % 8 != 0 && n % 8 <= 4;
+}
 constexpr int kGradMinN = 14;
 constexpr int kGradMaxN = 26;
 
@@ -279,12 +294,13 @@ __device__ __forceinline__ void grad_flip(double (&hr)[N], double (&hi)[N], cons
     flip_mem<N, MODE, double>(hr, hi, M + (size_t)i * N, sg);
 }
 
-// One wave per workgroup; blockIdx = item * wpi + w, item = matrix * ngroups + group.
+// One wave per workgroup; blockIdx = item * wpi + w, item = matrix * ngroups + group;
+// group g holds columns min(col0 + g G, N - G) onwards.
 // partial[(item * wpi + w) * N * G + row * G + col]. G = 4 Q columns in Q slots of 4.
 template <int N, int U, int G>
 __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restrict__ A,
-                                                               int64_t mstride, int ngroups, int c,
-                                                               int wpi, int iters,
+                                                               int64_t mstride, int ngroups,
+                                                               int col0, int c, int wpi, int iters,
                                                                cd *__restrict__ partial) {
   constexpr int Q = G / kGradG;
   constexpr int NB = N - 1;
@@ -301,7 +317,7 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
   const int item = blockIdx.x / wpi;
   const int w = blockIdx.x - item * wpi;
   const int b = item / ngroups, g = item - b * ngroups;
-  const int j0 = min(g * G, N - G);
+  const int j0 = min(col0 + g * G, N - G);
   const cd *src = A + (size_t)b * mstride;
   for (int e = lane; e < N * N; e += 64) {
     const int i = e / N, jl = e - i * N;
@@ -473,21 +489,32 @@ __global__ __launch_bounds__(64) TB_GRAD_ATTR void grad_kernel(const cd *__restr
   }
 }
 
-// grad[b][i][j] = sum_w partial[(b, owner group of j), w][i][j - j0], fixed order.
+// grad[b][i][j] = sum_w partial[(b, owner group of j), w][i][j - j0], fixed order. With a
+// tail launch (tail = 1), columns from G ngroups on come from its 4-column groups, stored
+// after the G-column ones.
 __global__ void grad_reduce_kernel(const cd *__restrict__ partial, int n, int G, int ngroups,
-                                   int wpi, int batch, cd *__restrict__ grad) {
+                                   int tail, int wpi, int batch, cd *__restrict__ grad) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (int64_t)batch * n * n) return;
   const int b = (int)(idx / ((int64_t)n * n));
   const int rem = (int)(idx - (int64_t)b * n * n);
   const int i = rem / n, j = rem - i * n;
-  const int g = min(j / G, ngroups - 1);
-  const int k = j - min(g * G, n - G);
-  const cd *src = partial + ((size_t)(b * ngroups + g) * wpi) * n * G + (size_t)i * G + k;
+  const cd *src;
+  int stride;
+  if (tail && j >= G * ngroups) {
+    stride = n * kGradG;
+    src = partial + (size_t)batch * ngroups * wpi * n * G + ((size_t)b * wpi) * stride +
+          (size_t)i * kGradG + (j - (n - kGradG));
+  } else {
+    const int g = min(j / G, ngroups - 1);
+    const int k = j - min(g * G, n - G);
+    stride = n * G;
+    src = partial + ((size_t)(b * ngroups + g) * wpi) * n * G + (size_t)i * G + k;
+  }
   double re = 0.0, im = 0.0;
 #pragma unroll 8
   for (int w = 0; w < wpi; ++w) {  // unrolled: the loads are in flight together
-    const cd v = src[(size_t)w * n * G];
+    const cd v = src[(size_t)w * stride];
     re += v.re;
     im += v.im;
   }
@@ -499,7 +526,7 @@ __global__ void grad_reduce_kernel(const cd *__restrict__ partial, int n, int G,
 namespace tbgr {
 
 struct GradPlan {
-  int c, wpi, iters, ngroups;
+  int c, wpi, iters, ngroups, tail;  // tail: one 4-column group per matrix after ngroups
 };
 
 // Chunks of 2^c ranks. Built in: c shrinks (not above 11) until the batch provides
@@ -508,9 +535,10 @@ struct GradPlan {
 inline GradPlan grad_plan(int n, int batch, int c_pref) {
   GradPlan p{};
   const int G = tb::grad_g_of(n);
-  p.ngroups = (n + G - 1) / G;
+  p.tail = tb::grad_tail4(n) ? 1 : 0;
+  p.ngroups = p.tail ? n / G : (n + G - 1) / G;
   const int nb = n - 1;
-  const int64_t items = (int64_t)batch * p.ngroups;
+  const int64_t items = (int64_t)batch * (p.ngroups + p.tail);
   int c;
   if (c_pref > 0) {
     c = c_pref;
@@ -519,7 +547,8 @@ inline GradPlan grad_plan(int n, int batch, int c_pref) {
     while ((items << want_bits) < 8192 && want_bits < 30) ++want_bits;
     c = std::min(nb - 6 - want_bits, 11);
   }
-  c = std::max(c, G == 8 ? TB_GRAD_G8_U : (n <= TB_GRAD_U4_MAX_N ? 4 : 3));
+  const int u4 = n <= TB_GRAD_U4_MAX_N ? 4 : 3;
+  c = std::max(c, G == 8 ? (p.tail ? std::max(TB_GRAD_G8_U, u4) : TB_GRAD_G8_U) : u4);
   c = std::min(c, nb - 7);
   p.c = c;
   const int task_bits = nb - c - 6;  // >= 1
@@ -536,7 +565,8 @@ bool grad_supported(int n) { return n >= tb::kGradMinN && n <= tb::kGradMaxN; }
 size_t grad_partial_bytes(int n, int batch, int c_pref) {
   if (!grad_supported(n) || batch <= 0) return 0;
   const GradPlan p = grad_plan(n, batch, c_pref);
-  return align256((size_t)batch * p.ngroups * p.wpi * n * tb::grad_g_of(n) * sizeof(tb::cd));
+  return align256((size_t)batch * p.wpi * n *
+                  (p.ngroups * tb::grad_g_of(n) + p.tail * tb::kGradG) * sizeof(tb::cd));
 }
 
 template <int N = tb::kGradMinN>
@@ -546,10 +576,22 @@ int32_t dispatch_grad(int n, const tb::cd *A, int64_t mstride, int batch, const 
     return tbh::fail(TB_ERANGE, "grad: no single-sweep kernel for n=%d", n);
   } else {
     if (n != N) return dispatch_grad<N + 1>(n, A, mstride, batch, p, partial, s);
+    constexpr int G = tb::grad_g<N>();
     const unsigned grid = (unsigned)((int64_t)batch * p.ngroups * p.wpi);
-    hipLaunchKernelGGL((tb::grad_kernel<N, tb::grad_u<N>(), tb::grad_g<N>()>), dim3(grid), dim3(64),
-                       0, s, A, mstride, p.ngroups, p.c, p.wpi, p.iters, partial);
-    return tbh::check_hip(hipGetLastError(), "grad_kernel");
+    hipLaunchKernelGGL((tb::grad_kernel<N, tb::grad_u<N>(), G>), dim3(grid), dim3(64), 0, s, A,
+                       mstride, p.ngroups, 0, p.c, p.wpi, p.iters, partial);
+    int32_t rc = tbh::check_hip(hipGetLastError(), "grad_kernel");
+    if (rc || !p.tail) return rc;
+    if constexpr (G == 8 && N % 8 != 0 && N % 8 <= 4) {
+      // the last N % 8 columns: one 4-column group per matrix (columns N-4..N-1)
+      tb::cd *tail_partial = partial + (size_t)batch * p.ngroups * p.wpi * N * G;
+      hipLaunchKernelGGL((tb::grad_kernel<N, tb::grad_u_of<N, tb::kGradG>(), tb::kGradG>),
+                         dim3((unsigned)((int64_t)batch * p.wpi)), dim3(64), 0, s, A, mstride,
+                         1, N - tb::kGradG, p.c, p.wpi, p.iters, tail_partial);
+      return tbh::check_hip(hipGetLastError(), "grad_kernel (4-column tail)");
+    } else {
+      return tbh::fail(TB_EINVAL, "grad: no 4-column tail for n=%d", n);
+    }
   }
 }
 
@@ -561,7 +603,7 @@ int32_t run_grad(int n, const tb::cd *A, int64_t mstride, int batch, int c_pref,
   if (rc) return rc;
   const int64_t total = (int64_t)batch * n * n;
   hipLaunchKernelGGL(tb::grad_reduce_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                     s, partial, n, tb::grad_g_of(n), p.ngroups, p.wpi, batch, grad);
+                     s, partial, n, tb::grad_g_of(n), p.ngroups, p.tail, p.wpi, batch, grad);
   return tbh::check_hip(hipGetLastError(), "grad_reduce_kernel");
 }
 
