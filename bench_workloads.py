@@ -71,11 +71,11 @@ def clock_during(fn, seconds):
 
 class GuanFlops:
     """Counts, inside the with-block, the multiplicity-aware algorithmic flops of every
-    tb_guan_perm_batch launch the calculators make: per item prod_d (e_d + 1) Guan codes,
-    each costing 2|T| + 6(sum t - 1) + 2 (|T| complex adds into the sums of the product
-    side's distinct modes, sum t - 1 complex multiplies, the accumulation), with e the
-    enumerated side's multiplicities and t the product side's. The count is summed on the
-    device and read once at the end."""
+    tb_guan_perm_batch and tb_guan_perm_outer launch the calculators make: per item (pair)
+    prod_d (e_d + 1) Guan codes, each costing 2|T| + 6(sum t - 1) + 2 (|T| complex adds
+    into the sums of the product side's distinct modes, sum t - 1 complex multiplies, the
+    accumulation), with e the enumerated side's multiplicities and t the product side's.
+    The count is summed on the device and read once at the end."""
 
     def __enter__(self):
         T = D.torch()
@@ -93,11 +93,28 @@ class GuanFlops:
             self.items += int(in_mult.shape[0])
             return self._orig(U, out_modes, out_mult, in_modes, in_mult, formula)
 
+        self._orig_outer = ops.guan_perm_outer
+
+        def counted_outer(U, out_modes, out_mult, in_modes, in_mult, formula=L.TB_FORMULA_RYSER):
+            # pair (j, i) enumerates the T row j when it has fewer codes than the S row i
+            f64 = T.float64
+            codes_t, codes_s = (out_mult.to(f64) + 1).prod(dim=1), (in_mult.to(f64) + 1).prod(dim=1)
+            dist_t, dist_s = (out_mult > 0).sum(dim=1).to(f64), (in_mult > 0).sum(dim=1).to(f64)
+            parts_t, parts_s = out_mult.sum(dim=1).to(f64), in_mult.sum(dim=1).to(f64)
+            enum_s = codes_s[None, :] * (2 * dist_t[:, None] + 6 * (parts_t[:, None] - 1) + 2)
+            enum_t = codes_t[:, None] * (2 * dist_s[None, :] + 6 * (parts_s[None, :] - 1) + 2)
+            self._acc += T.where(codes_t[:, None] < codes_s[None, :], enum_t, enum_s).sum()
+            self.launches += 1
+            self.items += int(out_mult.shape[0]) * int(in_mult.shape[0])
+            return self._orig_outer(U, out_modes, out_mult, in_modes, in_mult, formula)
+
         ops.guan_perm_batch = counted
+        ops.guan_perm_outer = counted_outer
         return self
 
     def __exit__(self, *exc):
         ops.guan_perm_batch = self._orig
+        ops.guan_perm_outer = self._orig_outer
         self.flops = float(self._acc.item())
         return False
 
@@ -387,14 +404,15 @@ def exact(checker, m=12, n=6, eta=0.5):
         return np.asarray(calc.calculate_distribution())
 
     T = D.torch()
-    with GuanFlops():  # warm: library, workspaces and the counter's own torch kernels
+    # warm-up run (library, workspaces), which also counts the launches' flops: the
+    # distribution is deterministic, so the timed run makes the same launches, uncounted
+    with GuanFlops() as gf:
         run()
     T.cuda.synchronize()
-    with GuanFlops() as gf:
-        t0 = time.perf_counter()
-        dist = run()
-        T.cuda.synchronize()
-        dt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dist = run()
+    T.cuda.synchronize()
+    dt = time.perf_counter() - t0
     perms = sum(math.comb(l + m - 1, l) * math.comb(n, l) for l in range(1, n + 1))
     t0 = time.perf_counter()
     ref = np.asarray(checker.uniform_losses_distribution(U, inp, eta, checker.perm_ryser))
@@ -423,20 +441,24 @@ def gcca(checker):
     m, n, S = 20, 10, 200
     U = checker.haar_unitary(m, 1)
     inp = [1] * n + [0] * (m - n)
-    # warm-up run of the full size with other draws: kernels loaded (the flop counter's
-    # too: a first use of a kernel loads its code object, tens of ms) and device memory
-    # pools grown (the first 200-sample run in a process takes ~4x longer than later ones)
-    with GuanFlops():
-        GeneralizedCliffordsSimulationStrategy(GPURyserPermanentCalculator(U),
-                                               rng=np.random.default_rng(1)).simulate(inp, S)
+    # warm-up run of the full size with other draws: kernels loaded (a first use of a
+    # kernel loads its code object, tens of ms) and device memory pools grown (the first
+    # 200-sample run in a process takes ~4x longer than later ones)
+    GeneralizedCliffordsSimulationStrategy(GPURyserPermanentCalculator(U),
+                                           rng=np.random.default_rng(1)).simulate(inp, S)
     strat = GeneralizedCliffordsSimulationStrategy(GPURyserPermanentCalculator(U),
                                                    rng=np.random.default_rng(0))
     D.torch().cuda.synchronize()
+    t0 = time.perf_counter()
+    samples = strat.simulate(inp, S)
+    D.torch().cuda.synchronize()
+    dt = time.perf_counter() - t0
+    # the flop count, outside the timed run: the same draws again (same seed, so the same
+    # layers and launches) with the counter on every launch
     with GuanFlops() as gf:
-        t0 = time.perf_counter()
-        samples = strat.simulate(inp, S)
-        D.torch().cuda.synchronize()
-        dt = time.perf_counter() - t0
+        again = GeneralizedCliffordsSimulationStrategy(GPURyserPermanentCalculator(U),
+                                                       rng=np.random.default_rng(0)).simulate(inp, S)
+    assert again == samples, "the counted run drew other samples than the timed run"
     ones = [1] * n + [0] * (m - n)
     r = GPURyserPermanentCalculator(U, ones, ones).compute_permanent()
     g = GPUGlynnGrayPermanentCalculator(U, ones, ones).compute_permanent()

@@ -146,6 +146,24 @@ int32_t tb_guan_perm_batch(const tb_complex128 *U, int32_t ldu, const int32_t *o
                            const int32_t *in_mult, int32_t in_max, int32_t batch,
                            int32_t formula, tb_complex128 *out, void *stream);
 
+/* All pairs of n_out output rows and n_in input rows in one launch:
+ *   out[j*n_in + i] = perm(U[t_j, s_i]), t_j = row j of (out_modes, out_mult) (out_max
+ *   slots), s_i = row i of (in_modes, in_mult) (in_max slots), same slot layout as
+ *   tb_guan_perm_batch.
+ * Replaces the per-layer permanent loop of GeneralizedCliffordsSimulationStrategy
+ * (generalized_cliffords_simulation_strategy.py:162-171 over :224-247: every candidate
+ * output r + e_o against every substate of the input), which calls compute_permanent
+ * once per pair.
+ * Each pair enumerates the row with fewer Guan codes, prod (m + 1) (the output row when
+ * it has strictly fewer; perm(U[t, s]) = perm(U^T[s, t])), so either side may be
+ * enumerated: max(out_max, in_max) <= min(TB_GUAN_MAX_ROWS, TB_GUAN_MAX_COLS), else
+ * TB_ERANGE. Every row on both sides holds the same number of particles (square effective
+ * matrices). n_out * n_in <= INT32_MAX. */
+int32_t tb_guan_perm_outer(const tb_complex128 *U, int32_t ldu, const int32_t *out_modes,
+                           const int32_t *out_mult, int32_t out_max, int32_t n_out,
+                           const int32_t *in_modes, const int32_t *in_mult, int32_t in_max,
+                           int32_t n_in, int32_t formula, tb_complex128 *out, void *stream);
+
 /* ---------------------------------------------------------------------------------
  * Clifford & Clifford submatrices permanents: all input-particle-deleted minors in one
  * sweep. Replaces BSCCRyserSubmatricesPermanentCalculator.compute_permanents

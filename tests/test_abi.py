@@ -65,6 +65,15 @@ def test_workspace_queries_are_host_only(lib):
     lambda l: l.tb_sweep_minors_batch(None, 41, 40, 41 * 40, 1, None, None, 0, None),
     lambda l: l.tb_perm_grad_batch(None, 50, 1, 2500, ctypes.c_void_p(8), None, None, 0, None),
     lambda l: l.tb_guan_perm_batch(None, 1, None, None, 1, None, None, 1, 1, 5, None, None),
+    lambda l: l.tb_guan_perm_outer(None, 1, None, None, 1, 1, None, None, 1, 1, 5, None, None),
+    # either side may be enumerated: a 33-slot output side exceeds the 32 input slots
+    lambda l: l.tb_guan_perm_outer(*[ctypes.c_void_p(8), 40] + [ctypes.c_void_p(8)] * 2 +
+                                   [33, 4] + [ctypes.c_void_p(8)] * 2 + [4, 4, 0] +
+                                   [ctypes.c_void_p(8), None]),
+    # more pairs than a launch indexes
+    lambda l: l.tb_guan_perm_outer(*[ctypes.c_void_p(8), 40] + [ctypes.c_void_p(8)] * 2 +
+                                   [4, 1 << 16] + [ctypes.c_void_p(8)] * 2 + [4, 1 << 16, 0] +
+                                   [ctypes.c_void_p(8), None]),
     # more scratch than the static array of the no-workspace variant holds
     lambda l: l.tb_glynn_batch_noalloc(ctypes.c_void_p(8), 40, 256, 1600, 0, ctypes.c_void_p(8), None),
 ])
@@ -79,6 +88,7 @@ def test_invalid_arguments_fail_before_launch(lib, call):
 def test_zero_batch_is_a_no_op(lib):
     assert lib.tb_glynn_batch(None, 5, 0, 25, 0, None, None, 0, None) == 0
     assert lib.tb_sweep_minors_batch(None, 5, 4, 20, 0, None, None, 0, None) == 0
+    assert lib.tb_guan_perm_outer(None, 1, None, None, 1, 0, None, None, 1, 7, 0, None, None) == 0
 
 
 def test_tuning_table_roundtrip_and_errors(lib):

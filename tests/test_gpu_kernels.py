@@ -354,9 +354,66 @@ def test_guan_outer_batch_uniform_side_and_cached_copies(T, oracle):
     assert np.array_equal(got, pairs.reshape(len(Tm), len(S)))
     ref = np.array([[oracle.perm_classic(U, s, t) for s in S] for t in Tm])
     assert np.max(np.abs(got - ref)) <= 1e-12
+    # T rows of one orientation only: all collision-free (none transposed), all with a
+    # collision (all transposed); one launch over the rows in order
+    for rows in (np.nonzero(codes_t == 2 ** k)[0], np.nonzero(codes_t < 2 ** k)[0]):
+        one = calc.compute_permanent_outer(S, Tm[rows]).cpu().numpy()
+        assert np.array_equal(one, got[rows])
     calc.matrix *= 0.5  # in place: same object, new content
     again = calc.compute_permanent_outer(S, Tm).cpu().numpy()
     assert np.max(np.abs(again - ref * 0.5 ** k)) <= 1e-12
+
+
+def test_guan_perm_outer_entry_equals_pairwise_batch(T, oracle):
+    """tb_guan_perm_outer against tb_guan_perm_batch over the same pairs, each pair run in
+    the orientation the all-pairs kernel picks (the row with fewer Guan codes enumerated;
+    U transposed when that is the output row), both formulas: a packed launch (20,000
+    pairs, four per wave, both orientations mixed inside waves) and a small one (one pair
+    per wave), with the output side padded to more slots than the input side. Spot pairs
+    against brute force."""
+    from theboss_amd import _device as D
+    from theboss_amd import _lib as L
+    from theboss_amd import ops
+    from theboss_amd.states import compact_batch
+
+    rng = np.random.default_rng(21)
+    m, k = 12, 5
+    U = oracle.haar_unitary(m, 77)
+    S = np.array([np.bincount(rng.integers(0, m, k), minlength=m) for _ in range(30)]
+                 + [np.bincount(rng.choice(m, k, replace=False), minlength=m) for _ in range(10)])
+    Tm = np.array([np.bincount(rng.integers(0, m, k), minlength=m) for _ in range(500)])
+    sm, su = compact_batch(S)
+    tm, tu = compact_batch(Tm)
+    pad = 9 - tm.shape[1]
+    tm, tu = np.pad(tm, ((0, 0), (0, pad))), np.pad(tu, ((0, 0), (0, pad)))
+    Ud, Utd = D.to_device_c128(U), D.to_device_c128(np.ascontiguousarray(U.T))
+
+    def dev(a):
+        return D.to_device_i32(np.ascontiguousarray(a))
+
+    codes_s, codes_t = np.prod(S + 1.0, axis=1), np.prod(Tm + 1.0, axis=1)
+    for n_out in (500, 7):
+        for formula in (L.TB_FORMULA_RYSER, L.TB_FORMULA_CHIN_HUH):
+            got = ops.guan_perm_outer(Ud, dev(tm[:n_out]), dev(tu[:n_out]), dev(sm), dev(su),
+                                      formula).cpu().numpy()
+            assert got.shape == (n_out, len(S))
+            j, i = np.divmod(np.arange(n_out * len(S)), len(S))
+            tr = codes_t[j] < codes_s[i]
+            assert tr.any() and (~tr).any()
+            want = np.empty(n_out * len(S), dtype=np.complex128)
+            for flag in (False, True):
+                idx = np.nonzero(tr == flag)[0]
+                if flag:  # the output row enumerated: perm(U^T[s, t])
+                    r = ops.guan_perm_batch(Utd, dev(sm[i[idx]]), dev(su[i[idx]]),
+                                            dev(tm[j[idx]]), dev(tu[j[idx]]), formula)
+                else:
+                    r = ops.guan_perm_batch(Ud, dev(tm[j[idx]]), dev(tu[j[idx]]),
+                                            dev(sm[i[idx]]), dev(su[i[idx]]), formula)
+                want[idx] = r.cpu().numpy()
+            err = np.max(np.abs(got.reshape(-1) - want)) / np.max(np.abs(want))
+            assert err <= 1e-13, (n_out, formula, err)
+    for jj, ii in zip(rng.integers(0, 7, 12), rng.integers(0, len(S), 12)):
+        assert abs(got[jj, ii] - oracle.perm_classic(U, S[ii], Tm[jj])) <= 1e-12
 
 
 def _occupations(rng, m, ks):

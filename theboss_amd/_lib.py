@@ -73,6 +73,8 @@ SIGNATURES = {
     "tb_glynn_batch": (_i32, [_vp, _i32, _i32, _i64, _i32, _vp, _vp, _sz, _vp]),
     "tb_glynn_batch_noalloc": (_i32, [_vp, _i32, _i32, _i64, _i32, _vp, _vp]),
     "tb_guan_perm_batch": (_i32, [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, _vp, _vp]),
+    "tb_guan_perm_outer": (
+        _i32, [_vp, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _i32, _i32, _vp, _vp]),
     "tb_cc_workspace_size": (_sz, [_i32, _i32, _i32, _i32]),
     "tb_cc_submatrices_batch": (
         _i32, [_vp, _i32, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _i32, _i32, _vp, _vp, _sz, _vp]),

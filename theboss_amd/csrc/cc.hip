@@ -19,7 +19,7 @@ int32_t run_grad(int n, const tb::cd *A, int64_t mstride, int batch, int c_pref,
 namespace tbg {
 int32_t run_guan(const tb::cd *U, int ldu, const int *out_modes, const int *out_mult, int out_max,
                  const int *in_modes, const int *in_mult, int in_max, int batch, int formula,
-                 int mode, tb::cd *out, hipStream_t s);
+                 int mode, tb::cd *out, hipStream_t s, int n_in = 0);
 }  // namespace tbg
 
 namespace tb {

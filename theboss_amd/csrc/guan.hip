@@ -218,14 +218,20 @@ __device__ __forceinline__ void guan_item_run(const GuanItemLds &it, int D, int 
   }
 }
 
-// LDS: Macc[in_max][64] cd (mode 1) | IPW item blocks | rr, uu [in_max][64] int8.
-template <int JB, int IPW>
+// LDS: Macc[in_max][64] cd (mode 1) | IPW item blocks | rr, uu [cap][64] int8.
+// Batch form (OUTER false): item b pairs input row b with output row b. All-pairs form
+// (OUTER true, mode 0): item b = j * n_in + i pairs output row j (n_out rows) with input
+// row i (n_in rows), and the row with fewer Guan codes is enumerated, as the batch callers
+// choose per item on the host: perm(U[t, s]) = perm(U^T[s, t]), so a transposed item
+// swaps the two rows' roles and reads U with swapped strides. Its slots then hold up to
+// max(in_max, out_max) entries on either side.
+template <int JB, int IPW, bool OUTER = false>
 __global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int ldu,
                                                   const int *__restrict__ out_modes,
                                                   const int *__restrict__ out_mult, int out_max,
                                                   const int *__restrict__ in_modes,
                                                   const int *__restrict__ in_mult, int in_max,
-                                                  int batch, int formula, int mode,
+                                                  int batch, int n_in, int formula, int mode,
                                                   cd *__restrict__ out) {
   constexpr int W = 64 / IPW;  // lanes per item in a packed wave
   extern __shared__ char smem[];
@@ -234,12 +240,49 @@ __global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int 
   const int b0 = blockIdx.x * IPW;
   const int b = b0 + g;
   const bool live = b < batch;
+  // Slot capacities of the enumerated (columns) and product (rows) sides: wave-uniform.
+  const int cap = OUTER ? max(in_max, out_max) : in_max;
+  const int pcap = OUTER ? cap : out_max;
+  const int *emode = in_modes, *emult = in_mult, *pmode = out_modes, *pmult = out_mult;
+  int ew = in_max, pw = out_max;
+  size_t urow = (size_t)ldu, ucol = 1;
+  if (live) {
+    if constexpr (OUTER) {
+      const int j = b / n_in, i = b - j * n_in;
+      const int *sm = in_modes + (size_t)i * in_max, *su = in_mult + (size_t)i * in_max;
+      const int *tm = out_modes + (size_t)j * out_max, *tu = out_mult + (size_t)j * out_max;
+      // Code counts prod (m + 1) in double, as the host computes them (exact below 2^53).
+      double cs = 1.0, ct = 1.0;
+      for (int q = 0; q < in_max; ++q) cs *= (double)(max(su[q], 0) + 1);
+      for (int q = 0; q < out_max; ++q) ct *= (double)(max(tu[q], 0) + 1);
+      if (ct < cs) {
+        emode = tm;
+        emult = tu;
+        ew = out_max;
+        pmode = sm;
+        pmult = su;
+        pw = in_max;
+        urow = 1;
+        ucol = (size_t)ldu;
+      } else {
+        emode = sm;
+        emult = su;
+        pmode = tm;
+        pmult = tu;
+      }
+    } else {
+      emode = in_modes + (size_t)b * in_max;
+      emult = in_mult + (size_t)b * in_max;
+      pmode = out_modes + (size_t)b * out_max;
+      pmult = out_mult + (size_t)b * out_max;
+    }
+  }
   cd *Macc = (cd *)smem;
   char *items = smem + (mode == 1 ? sizeof(cd) * (size_t)in_max * 64 : 0);
-  const size_t stride = guan_item_stride(out_max, in_max);
+  const size_t stride = guan_item_stride(pcap, cap);
   signed char *rr = (signed char *)(items + IPW * stride);
-  signed char *uu = rr + in_max * 64;
-  const GuanItemLds it = guan_item_at(items + g * stride, out_max, in_max);
+  signed char *uu = rr + cap * 64;
+  const GuanItemLds it = guan_item_at(items + g * stride, pcap, cap);
 
   // Compact the nonzero slots of both sides in slot order: the group's lanes read W slots
   // at a time and place each nonzero one by a ballot prefix count. In mode 1 a zero input
@@ -247,12 +290,12 @@ __global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int 
   const uint64_t gmask = (W == 64) ? ~0ull : (((1ull << W) - 1ull) << (g * W));
   const uint64_t below = (1ull << lane) - 1ull;
   int D = 0, J = 0, ss = 0, st = 0;
-  for (int q0 = 0; q0 < in_max; q0 += W) {
+  for (int q0 = 0; q0 < cap; q0 += W) {
     const int q = q0 + l;
     int m = 0, md = 0;
-    if (live && q < in_max) {
-      m = in_mult[(size_t)b * in_max + q];
-      md = in_modes[(size_t)b * in_max + q];
+    if (live && q < ew) {
+      m = emult[q];
+      md = emode[q];
       if (mode == 1 && m <= 0) out[(size_t)b * in_max + q] = {0.0, 0.0};
     }
     const uint64_t bal = __ballot(m > 0) & gmask;
@@ -265,12 +308,12 @@ __global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int 
     }
     D += __popcll(bal);
   }
-  for (int q0 = 0; q0 < out_max; q0 += W) {
+  for (int q0 = 0; q0 < pcap; q0 += W) {
     const int q = q0 + l;
     int m = 0, md = 0;
-    if (live && q < out_max) {
-      m = out_mult[(size_t)b * out_max + q];
-      md = out_modes[(size_t)b * out_max + q];
+    if (live && q < pw) {
+      m = pmult[q];
+      md = pmode[q];
     }
     const uint64_t bal = __ballot(m > 0) & gmask;
     if (m > 0) {
@@ -287,7 +330,7 @@ __global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int 
   // Gather X[j][d] = U[row mode j][column mode d] over the compact slots.
   for (int e = l; e < J * D; e += W) {
     const int j = e / D, d = e - j * D;
-    it.X[e] = U[(size_t)it.rmode[j] * ldu + it.cmode[d]];
+    it.X[e] = U[(size_t)it.rmode[j] * urow + (size_t)it.cmode[d] * ucol];
   }
   __syncthreads();
   uint64_t total = 1;
@@ -305,7 +348,7 @@ __global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int 
     if (bq >= batch) break;
     const int Dq = __shfl(D, q * W, 64), Jq = __shfl(J, q * W, 64);
     const int ssq = __shfl(ss, q * W, 64), stq = __shfl(st, q * W, 64);
-    const GuanItemLds itq = guan_item_at(items + q * stride, out_max, in_max);
+    const GuanItemLds itq = guan_item_at(items + q * stride, pcap, cap);
     uint64_t totq = 1;
     for (int d = 0; d < Dq; ++d) totq *= (uint64_t)(itq.s[d] + 1);
     guan_item_run<JB, 64>(itq, Dq, Jq, ssq, stq, totq, bq, in_max, lane, lane, formula, mode,
@@ -328,29 +371,41 @@ size_t guan_lds_bytes(int out_max, int in_max, int mode, int ipw) {
   return bytes;
 }
 
+// n_in > 0: the all-pairs form over batch = n_out * n_in items (mode 0 only).
 int32_t run_guan(const tb::cd *U, int ldu, const int *out_modes, const int *out_mult, int out_max,
                  const int *in_modes, const int *in_mult, int in_max, int batch, int formula,
-                 int mode, tb::cd *out, hipStream_t s) {
-  const size_t lds1 = guan_lds_bytes(out_max, in_max, mode, 1);
+                 int mode, tb::cd *out, hipStream_t s, int n_in = 0) {
+  const bool outer = n_in > 0;
+  // slot capacities: an all-pairs item may enumerate either side
+  const int ecap = outer ? std::max(in_max, out_max) : in_max;
+  const int pcap = outer ? ecap : out_max;
+  const size_t lds1 = guan_lds_bytes(pcap, ecap, mode, 1);
   if (lds1 > 64 * 1024)
     return tbh::fail(TB_ERANGE, "guan: out_max=%d in_max=%d need %zu bytes of LDS", out_max,
                      in_max, lds1);
-  const size_t lds4 = guan_lds_bytes(out_max, in_max, mode, 4);
-  const bool pack = batch >= kGuanPackMinBatch && out_max <= 16 && lds4 <= 32 * 1024;
-#define TB_GUAN_LAUNCH(JB, IPW, LDS)                                                         \
-  hipLaunchKernelGGL((tb::guan_kernel<JB, IPW>), dim3((batch + IPW - 1) / IPW), dim3(64), LDS, \
-                     s, U, ldu, out_modes, out_mult, out_max, in_modes, in_mult, in_max, batch, \
-                     formula, mode, out)
+  const size_t lds4 = guan_lds_bytes(pcap, ecap, mode, 4);
+  const bool pack = batch >= kGuanPackMinBatch && pcap <= 16 && lds4 <= 32 * 1024;
+#define TB_GUAN_LAUNCH(JB, IPW, LDS)                                                          \
+  do {                                                                                        \
+    if (outer)                                                                                \
+      hipLaunchKernelGGL((tb::guan_kernel<JB, IPW, true>), dim3((batch + IPW - 1) / IPW),     \
+                         dim3(64), LDS, s, U, ldu, out_modes, out_mult, out_max, in_modes,    \
+                         in_mult, in_max, batch, n_in, formula, 0, out);                      \
+    else                                                                                      \
+      hipLaunchKernelGGL((tb::guan_kernel<JB, IPW, false>), dim3((batch + IPW - 1) / IPW),    \
+                         dim3(64), LDS, s, U, ldu, out_modes, out_mult, out_max, in_modes,    \
+                         in_mult, in_max, batch, 0, formula, mode, out);                      \
+  } while (0)
   if (pack) {
-    if (out_max <= 8)
+    if (pcap <= 8)
       TB_GUAN_LAUNCH(8, 4, lds4);
     else
       TB_GUAN_LAUNCH(16, 4, lds4);
-  } else if (out_max <= 8)
+  } else if (pcap <= 8)
     TB_GUAN_LAUNCH(8, 1, lds1);
-  else if (out_max <= 16)
+  else if (pcap <= 16)
     TB_GUAN_LAUNCH(16, 1, lds1);
-  else if (out_max <= 32)
+  else if (pcap <= 32)
     TB_GUAN_LAUNCH(32, 1, lds1);
   else
     TB_GUAN_LAUNCH(48, 1, lds1);
@@ -379,4 +434,30 @@ extern "C" int32_t tb_guan_perm_batch(const tb_complex128 *U, int32_t ldu,
                      TB_GUAN_MAX_ROWS, in_max, TB_GUAN_MAX_COLS);
   return tbg::run_guan((const tb::cd *)U, ldu, out_modes, out_mult, out_max, in_modes, in_mult,
                        in_max, batch, formula, 0, (tb::cd *)out, (hipStream_t)stream);
+}
+
+extern "C" int32_t tb_guan_perm_outer(const tb_complex128 *U, int32_t ldu,
+                                      const int32_t *out_modes, const int32_t *out_mult,
+                                      int32_t out_max, int32_t n_out, const int32_t *in_modes,
+                                      const int32_t *in_mult, int32_t in_max, int32_t n_in,
+                                      int32_t formula, tb_complex128 *out, void *stream) {
+  if (n_out < 0 || n_in < 0) return tbh::fail(TB_EINVAL, "guan_outer: n_out=%d n_in=%d < 0", n_out, n_in);
+  if (n_out == 0 || n_in == 0) return TB_OK;
+  if ((int64_t)n_out * n_in > INT32_MAX)
+    return tbh::fail(TB_ERANGE, "guan_outer: n_out * n_in = %lld pairs > %d",
+                     (long long)n_out * n_in, INT32_MAX);
+  if (formula != TB_FORMULA_RYSER && formula != TB_FORMULA_CHIN_HUH)
+    return tbh::fail(TB_EINVAL, "guan_outer: unknown formula %d", formula);
+  if (!U || !out_modes || !out_mult || !in_modes || !in_mult || !out)
+    return tbh::fail(TB_EINVAL, "guan_outer: NULL pointer");
+  if (out_max < 1 || in_max < 1 || ldu < 1)
+    return tbh::fail(TB_EINVAL, "guan_outer: out_max=%d in_max=%d ldu=%d must be >= 1", out_max,
+                     in_max, ldu);
+  const int w = std::max(out_max, in_max);
+  if (w > TB_GUAN_MAX_ROWS || w > TB_GUAN_MAX_COLS)
+    return tbh::fail(TB_ERANGE, "guan_outer: out_max=%d, in_max=%d: either side may be "
+                     "enumerated, so both must be <= %d", out_max, in_max,
+                     std::min(TB_GUAN_MAX_ROWS, TB_GUAN_MAX_COLS));
+  return tbg::run_guan((const tb::cd *)U, ldu, out_modes, out_mult, out_max, in_modes, in_mult,
+                       in_max, n_out * n_in, formula, 0, (tb::cd *)out, (hipStream_t)stream, n_in);
 }

@@ -59,6 +59,22 @@ def guan_perm_batch(U, out_modes, out_mult, in_modes, in_mult, formula: int = L.
     return out
 
 
+def guan_perm_outer(U, out_modes, out_mult, in_modes, in_mult, formula: int = L.TB_FORMULA_RYSER):
+    """All pairs of output rows j and input rows i in one launch: an (n_out, n_in) tensor
+    holding perm(U[t_j, s_i]); each pair enumerates the row with fewer Guan codes."""
+    T = D.require_gpu()
+    U = _c128(U, "U")
+    out_modes, out_mult = _i32(out_modes, "out_modes"), _i32(out_mult, "out_mult")
+    in_modes, in_mult = _i32(in_modes, "in_modes"), _i32(in_mult, "in_mult")
+    n_out, n_in = out_modes.shape[0], in_modes.shape[0]
+    out = T.empty((n_out, n_in), dtype=T.complex128, device=U.device)
+    L.check(L.load().tb_guan_perm_outer(
+        D.ptr(U), U.shape[1], D.ptr(out_modes), D.ptr(out_mult), out_modes.shape[1], n_out,
+        D.ptr(in_modes), D.ptr(in_mult), in_modes.shape[1], n_in, formula, D.ptr(out),
+        D.stream_ptr(U.device)))
+    return out
+
+
 def cc_submatrices_batch(U, out_modes, out_mult, in_modes, in_mult, k: int,
                          collision_free: bool):
     """All input-deleted minors per item: result[b, d] = perm(U[t_b, s_b - e_d])."""

@@ -207,8 +207,10 @@ class _GuanGPUCalculator(BSPermanentCalculatorBase):
     def compute_permanent_outer(self, S: np.ndarray, Tm: np.ndarray):
         """All pairs at once: a (len(Tm), len(S)) complex128 device tensor holding
         perm(U[Tm[j], S[i]]) at [j, i], for S and Tm rows that all hold the same number
-        k >= 1 of particles. Each side is compacted once on the host; the per-pair mode
-        lists are gathered on the device. None when a side needs the Glynn fallback."""
+        k >= 1 of particles. The S side is compacted once (cached by content), the Tm side
+        on the device, and one tb_guan_perm_outer launch walks every pair, each over the
+        side with fewer Guan codes (as compute_permanent_batch_arrays chooses). None when
+        a side needs the Glynn fallback."""
         S = np.asarray(S, dtype=np.int64)
         Tm = np.asarray(Tm, dtype=np.int64)
         k = int(S[0].sum()) if len(S) else 0
@@ -216,7 +218,6 @@ class _GuanGPUCalculator(BSPermanentCalculatorBase):
             raise AttributeError("outer batch needs rows with one common particle number >= 1")
         T = D.require_gpu()
         dev = D.device()
-        ns, nt = S.shape[0], Tm.shape[0]
         codes_s, cs = _cached(_content_key("S", dev, S), lambda: (
             np.prod(S.astype(np.float64) + 1, axis=1),
             [D.to_device_i32(a, dev) for a in compact_batch(S)]))
@@ -226,38 +227,8 @@ class _GuanGPUCalculator(BSPermanentCalculatorBase):
                 or max(codes_s.max(), codes_t.max()) > 2.0 ** 28:
             return None
         U = np.asarray(self._matrix, dtype=np.complex128)
-        Ud, Utd = _cached(_content_key("U", dev, U), lambda: (
-            D.to_device_c128(U, dev), D.to_device_c128(np.ascontiguousarray(U.T), dev)))
-
-        def run(transposed, j, i):
-            (em, eu), (pm, pu) = ((ct, cs) if transposed else (cs, ct))
-            ei, pi = (j, i) if transposed else (i, j)
-            return ops.guan_perm_batch(Utd if transposed else Ud, pm[pi], pu[pi], em[ei],
-                                       eu[ei], self.formula)
-
-        if np.all(codes_s == codes_s[0]):
-            # One code count on the S side (e.g. substates of a collision-free input): the
-            # orientation is a property of the T row, so the pairs of each orientation are
-            # whole rows of the result and no pair mask is formed.
-            res = T.empty((nt, ns), dtype=T.complex128, device=dev)
-            flip_rows = codes_t < codes_s[0]
-            for transposed in (False, True):
-                js = np.nonzero(flip_rows == transposed)[0]
-                if js.size == 0:
-                    continue
-                js_d = T.from_numpy(js.astype(np.int64)).to(dev)
-                j = js_d.repeat_interleave(ns)
-                i = T.arange(ns, device=dev).repeat(js.size)
-                res[js_d] = run(transposed, j, i).reshape(js.size, ns)
-            return res
-        flip = T.from_numpy(codes_t[:, None] < codes_s[None, :]).to(dev).reshape(-1)
-        res = T.empty(nt * ns, dtype=T.complex128, device=dev)
-        for transposed in (False, True):
-            idx = T.nonzero(flip == transposed).reshape(-1)
-            if idx.numel() == 0:
-                continue
-            res[idx] = run(transposed, idx // ns, idx % ns)
-        return res.reshape(nt, ns)
+        Ud = _cached(_content_key("U", dev, U), lambda: D.to_device_c128(U, dev))
+        return ops.guan_perm_outer(Ud, ct[0], ct[1], cs[0], cs[1], self.formula)
 
     def compute_permanent(self) -> complex128:
         s, t = self._checked_states()

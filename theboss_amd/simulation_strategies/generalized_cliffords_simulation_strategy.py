@@ -42,6 +42,7 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
         self.current_key: Tuple[int, ...] = tuple()
         self.current_sample_probability = 1.0
         self._layer_cache = {}
+        self._device_weights = {}
 
     def set_new_matrix(self, new_matrix) -> None:
         self._bs_permanent_calculator.matrix = new_matrix
@@ -62,6 +63,7 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
         for s in input_substates(self.input_state):
             self._labeled_states[sum(s)].append(s)
         self._layer_cache = {}
+        self._device_weights = {}
         self.pmfs = {}
 
     def _particles_kept(self, samples: int) -> Optional[np.ndarray]:
@@ -174,8 +176,11 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
             if hasattr(calc, "compute_permanent_outer") else None
         if perms is not None:   # device tensor (P * m) x |S|: pmfs formed on the device
             T = D.torch()
-            probs = (perms.real ** 2 + perms.imag ** 2) / T.from_numpy(norm).to(perms.device)
-            pmfs = (probs @ T.from_numpy(weights).to(perms.device)).reshape(P, m).cpu().numpy()
+            wn = self._device_weights.get((k, perms.device))
+            if wn is None:   # weight over normaliser per substate, uploaded once per k
+                wn = T.from_numpy(weights / norm).to(perms.device)
+                self._device_weights[(k, perms.device)] = wn
+            pmfs = (T.view_as_real(perms).square().sum(-1) @ wn).reshape(P, m).cpu().numpy()
         else:
             ins = np.tile(S, (P * m, 1))               # item = (prefix * m + o) * |S| + i
             outs = np.repeat(outputs.reshape(P * m, m), ns, axis=0)
