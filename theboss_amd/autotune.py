@@ -16,6 +16,7 @@ best is not the one that serves thousands. The table is process-wide; save()/loa
 keep it across processes.
 """
 import json
+import math
 import time
 from typing import Dict, Iterable, List, Optional
 
@@ -96,12 +97,27 @@ def sweep_builtin(k: int, items: int) -> int:
 
 
 def glynn_builtin(n: int, batch: int) -> int:
+    """The FP64 kernels' built-in chunk (two waves per SIMD): ~8192 waves, then up to two
+    sizes larger while the rounds of 2 x 1024 waves stay as full, scored by round fill
+    times the share of a chunk left after seeding, (n - 2) 2 / (5 2^c)."""
     nb = n - 1
     want = 0
     while (batch << want) < 8192 and want < 30:
         want += 1
-    c = max(nb - 6 - want, min(7, nb - 8))
-    return max(2, min(c, 12, nb - 8))
+
+    def clamp(c):
+        return max(2, min(c, 12, nb - 8))
+
+    c0 = clamp(max(nb - 6 - want, min(7, nb - 8)))
+    best, best_c = -1.0, c0
+    for c in (c0, c0 + 1, c0 + 2):
+        if clamp(c) != c:
+            continue
+        rounds = batch * 2.0 ** (nb - c - 6) / 2048
+        score = rounds / math.ceil(rounds) * (1 - (n - 2) * 2.0 / (5 * 2.0 ** c))
+        if score > best + 1e-9:
+            best, best_c = score, c
+    return best_c
 
 
 def grad_columns(n: int) -> int:
