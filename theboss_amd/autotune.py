@@ -130,13 +130,27 @@ def grad_static_bits(n: int) -> int:
 
 
 def grad_builtin(n: int, batch: int) -> int:
+    """The gradient's built-in chunk: ~8192 waves, then up to two sizes larger while the
+    main launch's rounds of 2 x 1024 waves stay as full, scored by round fill times the
+    share of a chunk left after seeding, 4n(n - 1) / ((8n + 18G - 30) 2^c)."""
     nb = n - 1
     g = grad_columns(n)
     groups = (n + g - 1) // g
+    main = n // g if g == 8 and 0 < n % 8 <= 4 else groups  # the 4-column tail apart
     want = 0
     while (batch * groups << want) < 8192 and want < 30:
         want += 1
-    return min(max(min(nb - 6 - want, 11), grad_static_bits(n)), nb - 7)
+    low = grad_static_bits(n)
+    if main != groups:  # the tail launch's 4-column groups need their static bits too
+        low = max(low, 4 if n <= 19 else 3)
+    c0 = min(max(min(nb - 6 - want, 11), low), nb - 7)
+    best, best_c = -1.0, c0
+    for c in range(c0, min(c0 + 2, 11, nb - 7) + 1):
+        rounds = batch * main * 2.0 ** (nb - c - 6) / 2048
+        score = rounds / math.ceil(rounds) * (1 - 4.0 * n * (n - 1) / ((8 * n + 18 * g - 30) * 2.0 ** c))
+        if score > best + 1e-9:
+            best, best_c = score, c
+    return best_c
 
 
 def _candidates(centre: int, lo: int, hi: int, spread: int) -> List[int]:
