@@ -63,18 +63,25 @@ class GPUBSCCRyserSubmatricesPermanentCalculator(BSSubmatricesPermanentCalculato
 
     def compute_permanents(self) -> List[complex128]:
         s = as_occupation(self._input_state)
-        t = as_occupation(self._output_state)
-        k = int(s.sum())
-        if k == 1:
+        if int(s.sum()) == 1:
             return [complex128(v) for v in self._input_state]
-        res = self.compute_permanents_batch([s], [t])[0]
-        return [complex128(v) for v in res]
+        return [complex128(v) for v in self.minors_array()]
+
+    def minors_array(self) -> np.ndarray:
+        """compute_permanents as one complex128 array of length m (no per-entry list)."""
+        s = as_occupation(self._input_state)
+        if int(s.sum()) == 1:
+            return s.astype(np.complex128)
+        return self._minors([(s, as_occupation(self._output_state))])[0]
 
     def compute_permanents_batch(self, input_states, output_states) -> np.ndarray:
         """Minors for many (input, output) pairs on the same matrix; all pairs must have
         the same particle numbers. Returns len(pairs) x m."""
+        return self._minors([(as_occupation(s), as_occupation(t))
+                             for s, t in zip(input_states, output_states)])
+
+    def _minors(self, pairs) -> np.ndarray:
         U = np.asarray(self._matrix, dtype=np.complex128)
-        pairs = [(as_occupation(s), as_occupation(t)) for s, t in zip(input_states, output_states)]
         B = len(pairs)
         m = len(pairs[0][0])
         ks = {int(p[0].sum()) for p in pairs}
@@ -104,11 +111,21 @@ class GPUBSCCRyserSubmatricesPermanentCalculator(BSSubmatricesPermanentCalculato
             out_mult[b, :len(ot)] = ot
             in_modes[b] = np.arange(Dd)
             in_mult[b, :len(it)] = it
-        dev = D.device()
+        # the gathered matrix block and the four index arrays in one upload (each pageable
+        # upload waits for the stream): bytes, viewed as complex128 and int32 on the device
+        T = D.require_gpu()
+        ints = np.concatenate([out_modes.ravel(), out_mult.ravel(), in_modes.ravel(),
+                               in_mult.ravel()])
+        buf = T.from_numpy(np.concatenate([stack.view(np.uint8).ravel(),
+                                           ints.view(np.uint8)])).to(D.device())
+        cb = stack.nbytes  # a multiple of 16: the int32 view starts aligned
+        Vd = buf[:cb].view(T.complex128).view(B * J, Dd)
+        ints_d = buf[cb:].view(T.int32)
+        oj, dj = B * J, B * Dd
         minors = ops.cc_submatrices_batch(
-            D.to_device_c128(stack, dev), D.to_device_i32(out_modes, dev),
-            D.to_device_i32(out_mult, dev), D.to_device_i32(in_modes, dev),
-            D.to_device_i32(in_mult, dev), k, collision_free).cpu().numpy()
+            Vd, ints_d[:oj].view(B, J), ints_d[oj:2 * oj].view(B, J),
+            ints_d[2 * oj:2 * oj + dj].view(B, Dd), ints_d[2 * oj + dj:].view(B, Dd), k,
+            collision_free).cpu().numpy()
         for b, (_, (im, _)) in enumerate(comps):
             out[b, im] = minors[b, :len(im)]
         return out

@@ -77,10 +77,15 @@ class GeneralizedCliffordsBSimulationStrategy(SimulationStrategyInterface):
         self._current_input[self._working_input_state.pop(idx)] += 1
 
     def _compute_pmf(self) -> None:
+        """:69-92. The Laplace sum runs over the occupied input modes only (the reference's
+        other terms are products with s_i = 0)."""
         U = np.asarray(self._bs_permanent_calculator.matrix, dtype=np.complex128)
         calc = self._submatrices_calculator_class(U, self._current_input, self.r_sample)
-        minors = np.asarray(calc.compute_permanents(), dtype=np.complex128)
-        amp = U @ (self._current_input * minors)
+        as_array = getattr(calc, "minors_array", None)
+        minors = as_array() if as_array is not None else \
+            np.asarray(calc.compute_permanents(), dtype=np.complex128)
+        occ = np.flatnonzero(self._current_input)
+        amp = U[:, occ] @ (self._current_input[occ] * minors[occ])
         p = amp.real ** 2 + amp.imag ** 2
         self.pmf = p / p.sum()
 
