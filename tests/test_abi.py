@@ -136,6 +136,24 @@ def test_sweep_plan_fills_the_gpu_for_small_batches(lib):
     assert lib.tb_sweep_workspace_size(k, 256) == 256 * 16 * k * 16  # 4096 waves in total
 
 
+def test_gradient_plan_matches_its_python_mirror(lib):
+    """autotune.grad_builtin, the centre of the tuner's candidates, is the library's built-in
+    gradient chunk: the workspace holds batch x wpi x n x (columns of the groups) partial
+    sums with wpi = 2^min(n - 7 - c, 13), which fixes c. Shapes where the plan grows the
+    chunk past the ~8192-wave choice (while the rounds stay full) and where it does not."""
+    from theboss_amd import _lib, autotune
+
+    for n, B in ((14, 1000), (16, 256), (17, 64), (20, 1), (20, 16), (20, 64), (22, 32),
+                 (24, 16), (26, 4)):
+        lib.tb_tune_set(_lib.TB_TUNE_GRAD, n, 0, 0)
+        c = autotune.grad_builtin(n, B)
+        g = autotune.grad_columns(n)
+        cols = (n // 8) * 8 + 4 if g == 8 and 0 < n % 8 <= 4 else -(-n // g) * g
+        want = B * 2 ** min(n - 7 - c, 13) * n * cols * 16
+        assert lib.tb_perm_grad_workspace_size(n, B) == -(-want // 256) * 256, (n, B, c)
+    assert autotune.grad_builtin(20, 64) == 9  # 2048 waves of 2^9 steps, not 8192 of 2^7
+
+
 def test_sampler_workspace_follows_tuned_sweeps(lib):
     from theboss_amd import _lib, ops
 
