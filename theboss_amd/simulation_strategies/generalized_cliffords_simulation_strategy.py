@@ -77,11 +77,13 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
         value = self._rng.random(len(pmfs)) * cum[:, -1]
         return np.minimum((cum <= value[:, None]).sum(axis=1), pmfs.shape[1] - 1)
 
-    def _compute_missing(self, keys) -> None:
-        missing = {k for k in keys if k not in self.pmfs}
-        by_count = defaultdict(list)
-        for k in missing:
-            by_count[sum(k)].append(k)
+    def _compute_missing(self, keys, counts) -> None:
+        """Memoise the layers of the keys not seen yet, one launch per particle count
+        (counts[i] = sum(keys[i])); each new key once, in sorted order."""
+        by_count = defaultdict(set)
+        for k, c in zip(keys, counts):
+            if k not in self.pmfs:
+                by_count[c].add(k)
         for _, group in sorted(by_count.items()):
             self._calculate_layers(sorted(group))
 
@@ -95,7 +97,7 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
             if active.size == 0:
                 continue
             act_keys = [keys[i] for i in active]
-            self._compute_missing(act_keys)
+            self._compute_missing(act_keys, current[active].sum(axis=1).tolist())
             idx = self._draw(np.stack([self.pmfs[k] for k in act_keys]))
             current[active, idx] += 1
             keys = list(map(tuple, current.tolist()))
