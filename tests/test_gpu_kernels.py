@@ -364,6 +364,33 @@ def test_guan_outer_batch_uniform_side_and_cached_copies(T, oracle):
     assert np.max(np.abs(again - ref * 0.5 ** k)) <= 1e-12
 
 
+def test_guan_layer_rows_from_prefixes_equal_outer(T, oracle):
+    """compute_permanent_layer (GCC-A's output rows prefix + e_o built in compact form from
+    the prefixes on the host) against compute_permanent_outer on the same rows given
+    densely: k = 1 (empty prefixes), prefixes with collisions, substates with
+    multiplicities (both orientations), both formulas, to rounding (the layer rows' slots
+    are not in mode order); spot values against brute force."""
+    from theboss_amd.permanent_calculators import GPUChinHuhPermanentCalculator, GPURyserPermanentCalculator
+
+    rng = np.random.default_rng(33)
+    m = 9
+    U = oracle.haar_unitary(m, 55)
+    for k in (1, 3, 5):
+        S = np.array([np.bincount(rng.integers(0, m, k), minlength=m) for _ in range(12)])
+        Pf = np.array([np.bincount(rng.integers(0, m, k - 1), minlength=m) for _ in range(7)])
+        rows = (Pf[:, None, :] + np.eye(m, dtype=np.int64)[None]).reshape(-1, m)
+        for cls in (GPURyserPermanentCalculator, GPUChinHuhPermanentCalculator):
+            calc = cls(U)
+            got = calc.compute_permanent_layer(S, Pf).cpu().numpy()
+            want = calc.compute_permanent_outer(S, rows).cpu().numpy()
+            assert got.shape == (len(rows), len(S))
+            assert np.max(np.abs(got - want)) <= 1e-13 * max(np.max(np.abs(want)), 1e-300), (k, cls)
+        for r, i in zip(rng.integers(0, len(rows), 6), rng.integers(0, len(S), 6)):
+            assert abs(got[r, i] - oracle.perm_classic(U, S[i], rows[r])) <= 1e-12
+    with pytest.raises(AttributeError):
+        GPURyserPermanentCalculator(U).compute_permanent_layer(S, Pf[:, :-1])
+
+
 def test_guan_perm_outer_entry_equals_pairwise_batch(T, oracle):
     """tb_guan_perm_outer against tb_guan_perm_batch over the same pairs, each pair run in
     the orientation the all-pairs kernel picks (the row with fewer Guan codes enumerated;

@@ -230,6 +230,47 @@ class _GuanGPUCalculator(BSPermanentCalculatorBase):
         Ud = _cached(_content_key("U", dev, U), lambda: D.to_device_c128(U, dev))
         return ops.guan_perm_outer(Ud, ct[0], ct[1], cs[0], cs[1], self.formula)
 
+    def compute_permanent_layer(self, S: np.ndarray, prefixes: np.ndarray):
+        """A GCC-A layer's pairs: a (len(prefixes) * m, len(S)) complex128 device tensor
+        holding perm(U[prefix_p + e_o, S[i]]) at [p * m + o, i], for S rows of k >= 1
+        particles and prefixes of k - 1. The output rows are built in compact form from the
+        compacted prefixes (each prefix's modes, then mode o: one more particle where the
+        prefix holds o, else a new slot), uploaded once, and walked by one
+        tb_guan_perm_outer launch. A row's slots are then not in mode order, which changes
+        the walk's rounding only. None when a side needs the Glynn fallback."""
+        S = np.asarray(S, dtype=np.int64)
+        Pf = np.asarray(prefixes, dtype=np.int64)
+        k = int(S[0].sum()) if len(S) else 0
+        if k < 1 or np.any(S.sum(axis=1) != k) or Pf.ndim != 2 or Pf.shape[1] != S.shape[1] \
+                or np.any(Pf.sum(axis=1) != k - 1):
+            raise AttributeError("a layer needs S rows of k >= 1 particles and prefixes of k - 1")
+        D.require_gpu()
+        dev = D.device()
+        P, m = Pf.shape
+        codes_s, cs = _cached(_content_key("S", dev, S), lambda: (
+            np.prod(S.astype(np.float64) + 1, axis=1),
+            [D.to_device_i32(a, dev) for a in compact_batch(S)]))
+        if k > 1:
+            pm, pu = (np.asarray(a, dtype=np.int64) for a in compact_batch(Pf))
+        else:
+            pm, pu = np.zeros((P, 1), np.int64), np.zeros((P, 1), np.int64)
+        W = pm.shape[1]
+        o = np.arange(m)
+        hit = (pm[:, None, :] == o[None, :, None]) & (pu[:, None, :] > 0)  # P x m x W
+        rows = np.empty((2, P, m, W + 1), dtype=np.int32)  # modes, multiplicities
+        rows[0, :, :, :W] = pm[:, None, :]
+        rows[0, :, :, W] = o[None, :]
+        rows[1, :, :, :W] = pu[:, None, :] + hit
+        rows[1, :, :, W] = ~hit.any(axis=2)
+        codes_t = np.prod(rows[1].astype(np.float64) + 1, axis=2)
+        if max(cs[0].shape[1], W + 1) > min(L.TB_GUAN_MAX_COLS, L.TB_GUAN_MAX_ROWS) \
+                or max(codes_s.max(), codes_t.max()) > 2.0 ** 28:
+            return None
+        td = D.to_device_i32(rows.reshape(2, P * m, W + 1), dev)
+        U = np.asarray(self._matrix, dtype=np.complex128)
+        Ud = _cached(_content_key("U", dev, U), lambda: D.to_device_c128(U, dev))
+        return ops.guan_perm_outer(Ud, td[0], td[1], cs[0], cs[1], self.formula)
+
     def compute_permanent(self) -> complex128:
         s, t = self._checked_states()
         if len(s) == 0:

@@ -174,8 +174,12 @@ class GeneralizedCliffordsSimulationStrategy(SimulationStrategyInterface):
         outputs = np.asarray(keys, dtype=np.int64)[:, None, :] + eye[None, :, :]   # P x m x m
         P, ns = len(keys), S.shape[0]
         calc = self._bs_permanent_calculator
-        perms = calc.compute_permanent_outer(S, outputs.reshape(P * m, m)) \
-            if hasattr(calc, "compute_permanent_outer") else None
+        if hasattr(calc, "compute_permanent_layer"):
+            perms = calc.compute_permanent_layer(S, np.asarray(keys, dtype=np.int64))
+        elif hasattr(calc, "compute_permanent_outer"):
+            perms = calc.compute_permanent_outer(S, outputs.reshape(P * m, m))
+        else:
+            perms = None
         if perms is not None:   # device tensor (P * m) x |S|: pmfs formed on the device
             T = D.torch()
             wn = self._device_weights.get((k, perms.device))
