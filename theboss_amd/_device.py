@@ -20,12 +20,21 @@ def torch():
     return _torch
 
 
+_gpu_checked = False
+
+
 def require_gpu():
+    """torch, after checking once per process that a GPU is visible and the HIP library
+    loads (both raise otherwise; the check is not repeated on every call, since
+    torch.cuda.is_available() re-counts the devices each time)."""
+    global _gpu_checked
     t = torch()
-    if not t.cuda.is_available():
-        raise RuntimeError(
-            "theboss_amd needs a ROCm GPU (gfx950); torch.cuda.is_available() is False")
-    _lib.load()
+    if not _gpu_checked:
+        if not t.cuda.is_available():
+            raise RuntimeError(
+                "theboss_amd needs a ROCm GPU (gfx950); torch.cuda.is_available() is False")
+        _lib.load()
+        _gpu_checked = True
     return t
 
 
