@@ -364,6 +364,60 @@ def test_guan_outer_batch_uniform_side_and_cached_copies(T, oracle):
     assert np.max(np.abs(again - ref * 0.5 ** k)) <= 1e-12
 
 
+def test_guan_outer_eight_items_per_wave_and_large_item_fallback(T, oracle):
+    """All-pairs launches of rows with at most 9 slots and >= 16384 pairs put eight pairs in
+    a wave, 8 lanes each; a wave holding a pair of more than 512 Guan codes walks its pairs
+    one after another with all 64 lanes. Rows of 12 particles on 6 modes (2^6..4^6 codes
+    per row) mixed with rows of every multiplicity >= 1, so both paths run: against the
+    pairwise batch entry over the same pairs and the oracle's Ryser on a few pairs. Bar
+    1e-8 of the pair's value (floored at 1e-3 of the largest): at 12 particles with
+    multiplicities up to 5 the Ryser terms cancel (sum of |terms| over |perm| is 2e4..2.5e5
+    on typical pairs here), so other lane splits and the oracle's sequential walk round
+    differently by up to ~1e-9; a wrong walk, sign or lane mapping errs by O(1)."""
+    from theboss_amd import _device as D
+    from theboss_amd import _lib as L
+    from theboss_amd import ops
+    from theboss_amd.states import compact_batch
+
+    rng = np.random.default_rng(88)
+    m, k = 12, 12
+    U = oracle.haar_unitary(m, 12)
+    S = np.array([np.bincount(rng.integers(0, 6, k), minlength=m) for _ in range(30)]
+                 + [np.bincount(np.r_[np.arange(6), rng.integers(0, 6, k - 6)], minlength=m)
+                    for _ in range(10)])
+    Tm = np.array([np.bincount(6 + rng.integers(0, 6, k), minlength=m) for _ in range(500)])
+    codes_s, codes_t = np.prod(S + 1.0, axis=1), np.prod(Tm + 1.0, axis=1)
+    walk = np.minimum(codes_t[:, None], codes_s[None, :])
+    assert (walk > 512).any() and (walk <= 512).any()
+    sm, su = compact_batch(S)
+    tm, tu = compact_batch(Tm)
+    assert max(sm.shape[1], tm.shape[1]) <= 9 and len(S) * len(Tm) >= 16384
+    Ud, Utd = D.to_device_c128(U), D.to_device_c128(np.ascontiguousarray(U.T))
+
+    def dev(a):
+        return D.to_device_i32(np.ascontiguousarray(a))
+
+    got = ops.guan_perm_outer(Ud, dev(tm), dev(tu), dev(sm), dev(su), L.TB_FORMULA_RYSER)
+    got = got.cpu().numpy()
+    j, i = np.divmod(np.arange(len(Tm) * len(S)), len(S))
+    tr = codes_t[j] < codes_s[i]
+    want = np.empty(len(j), dtype=np.complex128)
+    for flag in (False, True):
+        idx = np.nonzero(tr == flag)[0]
+        if flag:
+            r = ops.guan_perm_batch(Utd, dev(sm[i[idx]]), dev(su[i[idx]]), dev(tm[j[idx]]),
+                                    dev(tu[j[idx]]), L.TB_FORMULA_RYSER)
+        else:
+            r = ops.guan_perm_batch(Ud, dev(tm[j[idx]]), dev(tu[j[idx]]), dev(sm[i[idx]]),
+                                    dev(su[i[idx]]), L.TB_FORMULA_RYSER)
+        want[idx] = r.cpu().numpy()
+    err = np.abs(got.reshape(-1) - want) / np.maximum(np.abs(want), 1e-3 * np.max(np.abs(want)))
+    assert err.max() <= 1e-8, err.max()
+    for jj, ii in zip(rng.integers(0, len(Tm), 4), rng.integers(0, len(S), 4)):
+        ref = oracle.perm_ryser(U, S[ii], Tm[jj])
+        assert abs(got[jj, ii] - ref) <= 1e-8 * max(abs(ref), 1e-3 * np.max(np.abs(want)))
+
+
 def test_guan_layer_rows_from_prefixes_equal_outer(T, oracle):
     """compute_permanent_layer (GCC-A's output rows prefix + e_o built in compact form from
     the prefixes on the host) against compute_permanent_outer on the same rows given

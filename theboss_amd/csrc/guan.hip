@@ -20,11 +20,11 @@
 // integer below 2^53, updated by exact division and multiplication with table binomials.
 //
 // Launch shapes. Large batches of small items (GCC-A layers: about 100 codes per item)
-// are latency-bound, so a wave takes IPW = 4 consecutive items: the lanes read the
-// items' occupation slots in parallel and compact them with ballots, and when every item
-// of the wave has at most kGuanPackedMaxCodes codes each item is walked by its own 16
-// lanes; otherwise the wave walks its items one after another with all 64 lanes. Small
-// batches keep one item per wave (IPW = 1, W = 64).
+// are latency-bound, so a wave takes IPW = 4 consecutive items (8 for all-pairs launches
+// of narrow rows): the lanes read the items' occupation slots in parallel and compact them
+// with ballots, and when every item of the wave has at most 64 codes per lane each item is
+// walked by its own W = 64 / IPW lanes; otherwise the wave walks its items one after
+// another with all 64 lanes. Small batches keep one item per wave (IPW = 1, W = 64).
 #include <algorithm>
 
 #include "tb_common.hpp"
@@ -40,9 +40,9 @@ __device__ __forceinline__ double binom_exact(int n, int k) {
   return rint(r);
 }
 
-// A packed wave walks each of its items with 16 lanes when every item has at most this
-// many Guan codes (at most 64 codes per lane).
-constexpr int kGuanPackedMaxCodes = 1024;
+// A packed wave walks each of its items with its own W = 64 / IPW lanes when every item
+// has at most this many Guan codes per lane (1024 codes at 16 lanes, 512 at 8).
+constexpr int kGuanPackedCodesPerLane = 64;
 
 // Per-item LDS block: X[J][D] (compact rows x compact columns) for at most out_max x
 // in_max, then the compact input multiplicities s, their slots and modes, and the compact
@@ -336,7 +336,7 @@ __global__ __launch_bounds__(64) void guan_kernel(const cd *__restrict__ U, int 
   uint64_t total = 1;
   for (int d = 0; d < D; ++d) total *= (uint64_t)(it.s[d] + 1);
 
-  if (IPW == 1 || __all(!live || total <= (uint64_t)kGuanPackedMaxCodes)) {
+  if (IPW == 1 || __all(!live || total <= (uint64_t)kGuanPackedCodesPerLane * W)) {
     if (live)
       guan_item_run<JB, W>(it, D, J, ss, st, total, b, in_max, l, lane, formula, mode, Macc, rr,
                            uu, out);
@@ -385,6 +385,12 @@ int32_t run_guan(const tb::cd *U, int ldu, const int *out_modes, const int *out_
                      in_max, lds1);
   const size_t lds4 = guan_lds_bytes(pcap, ecap, mode, 4);
   const bool pack = batch >= kGuanPackMinBatch && pcap <= 16 && lds4 <= 32 * 1024;
+  // All-pairs launches of rows with at most 9 slots a side (GCC-A layers up to 9 particles,
+  // the exact distributions: items of at most 2^9 codes when their multiplicities are 1)
+  // put eight items in a wave, 8 lanes each: a lane seeds its sums once per chunk, and at
+  // 16 lanes per item of 2^5..2^7 codes the seeds cost as much as the 2..8 codes walked.
+  const size_t lds8 = guan_lds_bytes(pcap, ecap, mode, 8);
+  const bool pack8 = outer && pack && ecap <= 9 && lds8 <= 32 * 1024;
 #define TB_GUAN_LAUNCH(JB, IPW, LDS)                                                          \
   do {                                                                                        \
     if (outer)                                                                                \
@@ -396,7 +402,12 @@ int32_t run_guan(const tb::cd *U, int ldu, const int *out_modes, const int *out_
                          dim3(64), LDS, s, U, ldu, out_modes, out_mult, out_max, in_modes,    \
                          in_mult, in_max, batch, 0, formula, mode, out);                      \
   } while (0)
-  if (pack) {
+  if (pack8) {
+    if (pcap <= 8)
+      TB_GUAN_LAUNCH(8, 8, lds8);
+    else
+      TB_GUAN_LAUNCH(16, 8, lds8);
+  } else if (pack) {
     if (pcap <= 8)
       TB_GUAN_LAUNCH(8, 4, lds4);
     else
