@@ -365,7 +365,7 @@ def test_guan_outer_batch_uniform_side_and_cached_copies(T, oracle):
 
 
 def test_guan_outer_eight_items_per_wave_and_large_item_fallback(T, oracle):
-    """All-pairs launches of rows with at most 9 slots and >= 16384 pairs put eight pairs in
+    """All-pairs launches of rows with at most 8 slots and >= 16384 pairs put eight pairs in
     a wave, 8 lanes each; a wave holding a pair of more than 512 Guan codes walks its pairs
     one after another with all 64 lanes. Rows of 12 particles on 6 modes (2^6..4^6 codes
     per row) mixed with rows of every multiplicity >= 1, so both paths run: against the
@@ -391,7 +391,7 @@ def test_guan_outer_eight_items_per_wave_and_large_item_fallback(T, oracle):
     assert (walk > 512).any() and (walk <= 512).any()
     sm, su = compact_batch(S)
     tm, tu = compact_batch(Tm)
-    assert max(sm.shape[1], tm.shape[1]) <= 9 and len(S) * len(Tm) >= 16384
+    assert max(sm.shape[1], tm.shape[1]) <= 8 and len(S) * len(Tm) >= 16384
     Ud, Utd = D.to_device_c128(U), D.to_device_c128(np.ascontiguousarray(U.T))
 
     def dev(a):

@@ -21,7 +21,7 @@
 //
 // Launch shapes. Large batches of small items (GCC-A layers: about 100 codes per item)
 // are latency-bound, so a wave takes IPW = 4 consecutive items (8 for all-pairs launches
-// of narrow rows): the lanes read the items' occupation slots in parallel and compact them
+// of rows with at most 8 slots): the lanes read the items' occupation slots in parallel and compact them
 // with ballots, and when every item of the wave has at most 64 codes per lane each item is
 // walked by its own W = 64 / IPW lanes; otherwise the wave walks its items one after
 // another with all 64 lanes. Small batches keep one item per wave (IPW = 1, W = 64).
@@ -385,12 +385,14 @@ int32_t run_guan(const tb::cd *U, int ldu, const int *out_modes, const int *out_
                      in_max, lds1);
   const size_t lds4 = guan_lds_bytes(pcap, ecap, mode, 4);
   const bool pack = batch >= kGuanPackMinBatch && pcap <= 16 && lds4 <= 32 * 1024;
-  // All-pairs launches of rows with at most 9 slots a side (GCC-A layers up to 9 particles,
-  // the exact distributions: items of at most 2^9 codes when their multiplicities are 1)
+  // All-pairs launches of rows with at most 8 slots a side (GCC-A layers up to 8 particles,
+  // the exact distributions: items of at most 2^8 codes when their multiplicities are 1)
   // put eight items in a wave, 8 lanes each: a lane seeds its sums once per chunk, and at
-  // 16 lanes per item of 2^5..2^7 codes the seeds cost as much as the 2..8 codes walked.
+  // 16 lanes per item of 2^5..2^7 codes the seeds cost as much as the 2..8 codes walked
+  // (GCC-A m = 20: 20-40% less kernel time at 3..7 particles, even at 8; at 9, items of 2^9
+  // codes, 8 lanes took 239 us against 187 with 16).
   const size_t lds8 = guan_lds_bytes(pcap, ecap, mode, 8);
-  const bool pack8 = outer && pack && ecap <= 9 && lds8 <= 32 * 1024;
+  const bool pack8 = outer && pack && ecap <= 8 && lds8 <= 32 * 1024;
 #define TB_GUAN_LAUNCH(JB, IPW, LDS)                                                          \
   do {                                                                                        \
     if (outer)                                                                                \
