@@ -426,3 +426,28 @@ def test_strategy_factory_types_and_copies(oracle):
     other = GPURyserPermanentCalculator(U * 0.5)
     f.bs_permanent_calculator = other
     assert f.bs_permanent_calculator is not other
+
+
+def test_layer_rows_are_prefix_plus_one_particle_in_each_mode():
+    """states.layer_rows, the compact output rows of a GCC-A layer (prefix + e_o for every
+    mode o), rebuilt as dense occupations: equal to the prefixes plus the identity, for
+    empty prefixes, prefixes with collisions and mixed widths; at most one slot per mode
+    carries particles."""
+    from theboss_amd.states import layer_rows
+
+    rng = np.random.default_rng(12)
+    m = 11
+    for k in (1, 2, 4, 7):
+        P = 30
+        Pf = np.array([np.bincount(rng.integers(0, m, k - 1), minlength=m) for _ in range(P)]) \
+            if k > 1 else np.zeros((P, m), dtype=np.int64)
+        rows = layer_rows(Pf)
+        assert rows.dtype == np.int32 and rows.shape[:2] == (2, P * m)
+        dense = np.zeros((P * m, m), dtype=np.int64)
+        np.add.at(dense, (np.repeat(np.arange(P * m), rows.shape[2]), rows[0].ravel()),
+                  rows[1].ravel())
+        want = (Pf[:, None, :] + np.eye(m, dtype=np.int64)[None]).reshape(-1, m)
+        assert np.array_equal(dense, want), k
+        for r in range(P * m):
+            used = rows[0, r][rows[1, r] > 0]
+            assert len(used) == len(set(used.tolist()))

@@ -21,7 +21,7 @@ from numpy import complex128
 from .. import _device as D
 from .. import _lib as L
 from .. import ops
-from ..states import as_occupation, compact_batch, effective_matrix
+from ..states import as_occupation, compact_batch, effective_matrix, layer_rows
 from .bs_permanent_calculator_interface import BSPermanentCalculatorInterface
 
 
@@ -246,27 +246,15 @@ class _GuanGPUCalculator(BSPermanentCalculatorBase):
             raise AttributeError("a layer needs S rows of k >= 1 particles and prefixes of k - 1")
         D.require_gpu()
         dev = D.device()
-        P, m = Pf.shape
         codes_s, cs = _cached(_content_key("S", dev, S), lambda: (
             np.prod(S.astype(np.float64) + 1, axis=1),
             [D.to_device_i32(a, dev) for a in compact_batch(S)]))
-        if k > 1:
-            pm, pu = (np.asarray(a, dtype=np.int64) for a in compact_batch(Pf))
-        else:
-            pm, pu = np.zeros((P, 1), np.int64), np.zeros((P, 1), np.int64)
-        W = pm.shape[1]
-        o = np.arange(m)
-        hit = (pm[:, None, :] == o[None, :, None]) & (pu[:, None, :] > 0)  # P x m x W
-        rows = np.empty((2, P, m, W + 1), dtype=np.int32)  # modes, multiplicities
-        rows[0, :, :, :W] = pm[:, None, :]
-        rows[0, :, :, W] = o[None, :]
-        rows[1, :, :, :W] = pu[:, None, :] + hit
-        rows[1, :, :, W] = ~hit.any(axis=2)
-        codes_t = np.prod(rows[1].astype(np.float64) + 1, axis=2)
-        if max(cs[0].shape[1], W + 1) > min(L.TB_GUAN_MAX_COLS, L.TB_GUAN_MAX_ROWS) \
+        rows = layer_rows(Pf)  # modes, multiplicities: 2 x (P m) x (W + 1)
+        codes_t = np.prod(rows[1].astype(np.float64) + 1, axis=1)
+        if max(cs[0].shape[1], rows.shape[2]) > min(L.TB_GUAN_MAX_COLS, L.TB_GUAN_MAX_ROWS) \
                 or max(codes_s.max(), codes_t.max()) > 2.0 ** 28:
             return None
-        td = D.to_device_i32(rows.reshape(2, P * m, W + 1), dev)
+        td = D.to_device_i32(rows, dev)
         U = np.asarray(self._matrix, dtype=np.complex128)
         Ud = _cached(_content_key("U", dev, U), lambda: D.to_device_c128(U, dev))
         return ops.guan_perm_outer(Ud, td[0], td[1], cs[0], cs[1], self.formula)

@@ -48,6 +48,29 @@ def compact_batch(states: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
     return modes, mult
 
 
+def layer_rows(prefixes: np.ndarray) -> np.ndarray:
+    """The occupations prefix_p + e_o (o < m) of a GCC-A layer in compact form, built from
+    the compacted prefixes: rows[0] modes, rows[1] multiplicities, row p * m + o of shape
+    (2, P * m, W + 1) int32 with W the prefixes' compact width. Slots 0..W-1 hold the
+    prefix's modes (one more particle where the mode is o), slot W holds o with
+    multiplicity 1 when the prefix does not hold it, else 0. Slots are not in mode order."""
+    Pf = np.asarray(prefixes, dtype=np.int64)
+    P, m = Pf.shape
+    if int(Pf.sum(axis=1).max(initial=0)) > 0:
+        pm, pu = (np.asarray(a, dtype=np.int64) for a in compact_batch(Pf))
+    else:
+        pm, pu = np.zeros((P, 1), np.int64), np.zeros((P, 1), np.int64)
+    W = pm.shape[1]
+    o = np.arange(m)
+    hit = (pm[:, None, :] == o[None, :, None]) & (pu[:, None, :] > 0)  # P x m x W
+    rows = np.empty((2, P, m, W + 1), dtype=np.int32)
+    rows[0, :, :, :W] = pm[:, None, :]
+    rows[0, :, :, W] = o[None, :]
+    rows[1, :, :, :W] = pu[:, None, :] + hit
+    rows[1, :, :, W] = ~hit.any(axis=2)
+    return rows.reshape(2, P * m, W + 1)
+
+
 def effective_matrix(matrix, input_state, output_state) -> np.ndarray:
     """M[a][b] = U[out_mode(a)][in_mode(b)], rows repeated t_j times and columns s_i
     times (boson_sampling_utilities.py:595-626). Empty (0 x 0) when either side has no
